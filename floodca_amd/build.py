@@ -1,0 +1,98 @@
+"""Builds the in-tree native libraries.
+
+* floodca_amd/libfloodca_hip.so, libfloodca_hip_f64.so -- the product: the HIP
+  kernels (hipcc, gfx950) and the C host (gcc), one library per precision.
+* tests/emu/libwalk_emu_{f32,f64}.so -- host emulation of the fused walk for the
+  CPU test suite (g++), never loaded by the product.
+* oracle/ -- the checker (oracle/build_oracle.sh).
+
+Run ``python -m floodca_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "floodca_amd")
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+INC = os.path.join(ROOT, "include")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+ARCH = os.environ.get("FCA_OFFLOAD_ARCH", "gfx950")
+
+# Contraction off everywhere: the step must round like the reference build.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
+             "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result", f"-I{INC}", f"-I{CSRC}"]
+C_FLAGS = ["-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+           "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{INC}", f"-I{CSRC}"]
+EMU_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+             "-Wall", "-Wno-unused-variable", f"-I{INC}", f"-I{CSRC}"]
+
+PRECS = {"f32": ([], "libfloodca_hip.so"), "f64": (["-DFCA_DOUBLE"], "libfloodca_hip_f64.so")}
+
+
+def lib_path(prec: str = "f32") -> str:
+    return os.path.join(PKG, PRECS[prec][1])
+
+
+def _run(cmd, quiet=False):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(ln for ln in r.stdout.splitlines()
+                    if "packed-fp32-ops' is not a recognized feature" not in ln)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    if out.strip() and not quiet:
+        print(out)
+    return out
+
+
+def build_hip(prec: str, extra=()) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    defs, name = PRECS[prec]
+    kobj = os.path.join(BUILD, f"fca_kernels_{prec}.o")
+    cobj = os.path.join(BUILD, f"hip_backend_{prec}.o")
+    hipcc = os.path.join(ROCM, "bin", "hipcc")
+    _run([hipcc, *HIP_FLAGS, *defs, *extra, "-c", os.path.join(CSRC, "fca_kernels.hip"),
+          "-o", kobj])
+    _run(["gcc", *C_FLAGS, *defs, "-c", os.path.join(CSRC, "hip_backend.c"), "-o", cobj])
+    out = lib_path(prec)
+    tmp = out + ".tmp"
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", kobj, cobj, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_emu(prec: str) -> str:
+    defs, _ = PRECS[prec]
+    out = os.path.join(ROOT, "tests", "emu", f"libwalk_emu_{prec}.so")
+    _run(["g++", *EMU_FLAGS, *defs, os.path.join(ROOT, "tests", "emu", "walk_emu.cpp"),
+          "-o", out, "-lm"])
+    return out
+
+
+def build_oracle() -> None:
+    _run(["bash", os.path.join(ROOT, "oracle", "build_oracle.sh")])
+
+
+def build_all(verbose: bool = True) -> list[str]:
+    if shutil.which(os.path.join(ROCM, "bin", "hipcc")) is None:
+        raise RuntimeError(f"hipcc not found under {ROCM}")
+    jobs = [(build_hip, "f32"), (build_hip, "f64"), (build_emu, "f32"), (build_emu, "f64")]
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        futs = [ex.submit(f, p) for f, p in jobs] + [ex.submit(build_oracle)]
+        outs = [f.result() for f in futs]
+    if verbose:
+        for o in outs:
+            if o:
+                print("built", os.path.relpath(o, ROOT))
+    return [o for o in outs if o]
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,484 @@
+/*
+ * fca_device.h -- scalar arithmetic of the WCA step shared by the HIP kernels
+ * (compiled for gfx950) and by the host-side emulation of the fused walk used
+ * in the CPU test suite. Every expression keeps the reference's operation
+ * order (src/kernels/physics.h, src/common/model.c) and is compiled with
+ * contraction off, so a float build reproduces the reference float build bit
+ * for bit wherever the libm calls agree.
+ */
+#ifndef FCA_DEVICE_H
+#define FCA_DEVICE_H
+
+#include "floodca_hip.h"
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define FCA_HD __host__ __device__ inline
+#else
+#define FCA_HD inline
+#endif
+
+/* Mask value used for lanes whose column lies outside the grid. */
+#define FCA_NONE 3
+
+/* Run-constant inputs of the step controller (C-visible). */
+typedef struct FcaCtlParams {
+    double T_sim, out_every, D_h, phi;
+    int hyeto, steps, U, n_force_days;
+} FcaCtlParams;
+
+/* Controller state carried from step to step on the device (C-visible). */
+typedef struct FcaDevCtrl {
+    double t;              /* time at the start of the next step */
+    long long dt_min_hits; /* dt_base clamped up to dt_min */
+    int step;              /* steps taken so far */
+    int force_day;         /* forcing day whose Rday is on the device, -1 none */
+    int state;             /* FCA_CTL_RUN, _DONE or _PAUSE */
+    int want_day;          /* with _PAUSE: the day to load */
+    int nrec;              /* step records written */
+    int hbuf;              /* which of the two depth buffers holds h */
+    double dt_base;        /* held step, in the run's precision */
+} FcaDevCtrl;
+
+enum { FCA_CTL_RUN = 0, FCA_CTL_DONE = 1, FCA_CTL_PAUSE = 2 };
+
+/* Per-workgroup balance terms, accumulated over the steps of a run. */
+typedef struct FcaAccSlot {
+    double v_rain, clamp, v_sea, v_bnd;
+    long long cap;
+    long long pad;
+} FcaAccSlot;
+
+#ifdef __cplusplus
+
+/* Physical constants of one run in the kernel's precision, plus the derived
+ * widths of physics.h:37-48. */
+template <class R>
+struct PhysK {
+    R dx, A, g, alpha, tau_h, tau_l, z_sea, dt_max, dt_min, S_bnd;
+    R w;     /* face width: dx (K=4) or (sqrt2-1)*dx on every face (K=8) */
+    R ddiag; /* sqrt2*dx */
+    int U, K, open;
+};
+
+/* hf^(2/3) as (real)cbrt((double)hf*(double)hf) (physics.h:19-24). */
+FCA_HD float cbrt_sq(float hf)
+{
+    double sq = (double)hf * (double)hf;
+    return (float)cbrt(sq);
+}
+FCA_HD double cbrt_sq(double hf)
+{
+    double sq = hf * hf;
+    return cbrt(sq);
+}
+
+/* (real)sqrt((double)x): the double detour the reference takes in float mode
+ * (physics.h:106,205). */
+FCA_HD float sqrt_via_double(float x) { return (float)sqrt((double)x); }
+FCA_HD double sqrt_via_double(double x) { return sqrt(x); }
+
+/* (real)sqrt((double)g*(double)h), the critical velocity (physics.h:108,207). */
+FCA_HD float crit_speed(float g, float h) { return (float)sqrt((double)g * (double)h); }
+FCA_HD double crit_speed(double g, double h) { return sqrt(g * h); }
+
+/* Face flux a->b, positive = a to b (physics.h:51-125). den = (real)(2K)*dt,
+ * evaluated once per step by the caller exactly as the reference evaluates it
+ * per face. */
+template <class R>
+FCA_HD R face_flux(R za, R ha, R na, int ma, R zb, R hb, R nb, int mb, R d, R dt, R den,
+                   const PhysK<R> &p, int &cap)
+{
+    R bed_a, bed_b, lev_a, lev_b, dL, adL, hf, nf, v, vcr, qmag;
+    cap = 0;
+    if (ma == FCA_WALL || mb == FCA_WALL) {
+        return (R)0;
+    }
+    bed_a = (ma == FCA_SEA) ? p.z_sea : za;
+    bed_b = (mb == FCA_SEA) ? p.z_sea : zb;
+    lev_a = bed_a + ((ma == FCA_SEA) ? (R)0 : ha);
+    lev_b = bed_b + ((mb == FCA_SEA) ? (R)0 : hb);
+    dL = lev_a - lev_b;
+    if (dL < (R)0) {
+        if (-dL < p.tau_l) {
+            return (R)0;
+        }
+    } else if (dL < p.tau_l) {
+        return (R)0;
+    }
+    hf = ((lev_a > lev_b) ? lev_a : lev_b) - ((bed_a > bed_b) ? bed_a : bed_b);
+    if (hf < p.tau_h) {
+        return (R)0;
+    }
+    if (((dL > (R)0) ? ma : mb) == FCA_SEA) {
+        return (R)0;
+    }
+    if (ma == FCA_SEA) {
+        nf = nb;
+    } else if (mb == FCA_SEA) {
+        nf = na;
+    } else {
+        nf = (R)0.5 * (na + nb);
+    }
+    if (nf < (R)1e-12) {
+        nf = (R)1e-12;
+    }
+    adL = (dL < (R)0) ? -dL : dL;
+    v = ((R)1 / nf) * cbrt_sq(hf) * sqrt_via_double(adL / d);
+    vcr = crit_speed(p.g, hf);
+    if (v > vcr) {
+        v = vcr;
+    }
+    qmag = v * hf * p.w;
+    if (dt > (R)0 && p.K > 0) {
+        R qcap = adL * p.A / den;
+        if (qcap < qmag) {
+            cap = 1;
+            qmag = qcap;
+        }
+    }
+    return (dL > (R)0) ? qmag : -qmag;
+}
+
+/* Open-boundary virtual face of a LAND edge cell: z_b = z_a - S_bnd*dx,
+ * h_b = 0, n_b = n_a, d = dx; inflow clipped to 0 (model.c:166-177). */
+template <class R>
+FCA_HD R open_flux(R za, R ha, R na, R dt, R den, const PhysK<R> &p, int &cap)
+{
+    R zb = za - p.S_bnd * p.dx;
+    R q = face_flux<R>(za, ha, na, FCA_LAND, zb, (R)0, na, FCA_LAND, p.dx, dt, den, p, cap);
+    return (q < (R)0) ? (R)0 : q;
+}
+
+/* Limiter factor (physics.h:127-134). */
+template <class R>
+FCA_HD R limiter_s(R out_vol, R h, R A)
+{
+    R hold = h * A;
+    if (out_vol > hold && out_vol > (R)0) {
+        return hold / out_vol;
+    }
+    return (R)1;
+}
+
+/* |la - lb| / d (physics.h:147-154). */
+template <class R>
+FCA_HD R abs_slope(R la, R lb, R d)
+{
+    R dL = la - lb;
+    if (dL < (R)0) {
+        dL = -dL;
+    }
+    return dL / d;
+}
+
+/* Level a neighbour presents to the slope test (physics.h:136-145). */
+template <class R>
+FCA_HD R level_of(int m, R z, R h, const PhysK<R> &p)
+{
+    return (m == FCA_SEA) ? p.z_sea : ((m == FCA_WALL) ? z : z + h);
+}
+
+/* Stable step of a wet LAND cell from its largest slope
+ * (physics.h:193-217); the caller has checked h >= tau_h. */
+template <class R>
+FCA_HD R cell_dt_from_smax(R h, R n, R smax, const PhysK<R> &p)
+{
+    R v, vcr;
+    if (h < p.tau_h) {
+        return p.dt_max;
+    }
+    if (n < (R)1e-12) {
+        n = (R)1e-12;
+    }
+    if (smax < (R)0) {
+        smax = (R)0;
+    }
+    v = ((R)1 / n) * cbrt_sq(h) * sqrt_via_double(smax);
+    vcr = crit_speed(p.g, h);
+    if (v > vcr) {
+        v = vcr;
+    }
+    if (v < (R)1e-6) {
+        v = (R)1e-6;
+    }
+    return p.alpha * p.dx / v;
+}
+
+/* ---- the step controller (model.c:363-382, 599-601; rain.c:13-106) ------ */
+
+/* What one launch does, derived from FcaDevCtrl and the time-step candidate. */
+template <class R>
+struct StepVals {
+    double t, F, t_new;
+    R dt, dt_base, tmax_val, rain_f, den;
+    int active, hit, need_cand, step;
+};
+
+/* rain.c:13-28 */
+FCA_HD double tri_rate(double t, double T, double tp)
+{
+    if (t < 0.0 || t >= T || T <= 0.0) {
+        return 0.0;
+    }
+    if (t < tp) {
+        if (tp <= 0.0) {
+            return 0.0;
+        }
+        return (2.0 / T) * (t / tp);
+    }
+    if (T - tp <= 0.0) {
+        return 0.0;
+    }
+    return (2.0 / T) * ((T - t) / (T - tp));
+}
+
+/* rain.c:30-67; D_h and phi arrive already widened from real. */
+FCA_HD double hyeto_integral(const FcaCtlParams &c, double t, double dt)
+{
+    double t1, T, tp, sum, a, b;
+    if (dt <= 0.0) {
+        return 0.0;
+    }
+    t1 = t + dt;
+    if (c.hyeto == FCA_HYETO_UNIFORM) {
+        return dt / 86400.0;
+    }
+    T = c.D_h * 3600.0;
+    tp = c.phi * T;
+    if (t1 <= 0.0 || t >= T) {
+        return 0.0;
+    }
+    if (t < 0.0) {
+        t = 0.0;
+    }
+    if (t1 > T) {
+        t1 = T;
+    }
+    sum = 0.0;
+    a = t;
+    while (a < t1 - 1e-15) {
+        b = (a < tp && t1 > tp) ? tp : t1;
+        sum += 0.5 * (tri_rate(a, T, tp) + tri_rate(b, T, tp)) * (b - a);
+        a = b;
+    }
+    return sum;
+}
+
+/* rain.c:69-93 */
+FCA_HD double next_event_time(const FcaCtlParams &c, double t)
+{
+    double t_next = c.T_sim, t_day, t_out;
+    t_day = (floor(t / FCA_DAY_S) + 1.0) * FCA_DAY_S;
+    if (t_day - t < 1e-9) {
+        t_day += FCA_DAY_S;
+    }
+    if (t_day < t_next) {
+        t_next = t_day;
+    }
+    if (c.out_every > 0.0) {
+        t_out = (floor(t / c.out_every) + 1.0) * c.out_every;
+        if (t_out - t < 1e-9) {
+            t_out += c.out_every;
+        }
+        if (t_out < t_next) {
+            t_next = t_out;
+        }
+    }
+    return t_next;
+}
+
+/* rain.c:95-106 */
+FCA_HD int forcing_day_index(double t)
+{
+    int d;
+    if (t < 0.0) {
+        return 0;
+    }
+    d = (int)floor(t / FCA_DAY_S);
+    return d < 0 ? 0 : d;
+}
+
+/* One evaluation of fca_step's head (model.c:338-382) and tail (599-601).
+ * cand is the smallest per-cell stable step of the current state (+inf when
+ * no LAND cell is wet). Fills sv for this launch and nx for the next one. A
+ * launch is inactive when the run is over, paused for a forcing reload, or
+ * the event clamp leaves no time to step (model.c:379-381). */
+template <class R>
+FCA_HD void ctrl_eval(const FcaDevCtrl &in, R cand, const FcaCtlParams &c, const PhysK<R> &p,
+                      StepVals<R> &sv, FcaDevCtrl &nx)
+{
+    R dtb, dt;
+    double t_next, remain;
+    int hit = 0;
+    nx = in;
+    sv.active = 0;
+    sv.need_cand = 0;
+    sv.step = in.step;
+    sv.t = in.t;
+    if (in.state != FCA_CTL_RUN) {
+        return;
+    }
+    if (c.steps > 0 && in.step >= c.steps) {
+        return;
+    }
+    if (c.steps <= 0 && in.t >= c.T_sim - 1e-15) {
+        nx.state = FCA_CTL_DONE;
+        return;
+    }
+    if (c.n_force_days > 0) {
+        int d = forcing_day_index(in.t);
+        if (d >= c.n_force_days) {
+            d = c.n_force_days - 1;
+        }
+        if (d != in.force_day) {
+            nx.state = FCA_CTL_PAUSE;
+            nx.want_day = d;
+            return;
+        }
+    }
+    dtb = (R)in.dt_base;
+    if (in.step % (c.U > 0 ? c.U : 1) == 0) {
+        R dtmin = p.dt_max;
+        if (cand < dtmin) {
+            dtmin = cand;
+        }
+        if (dtmin < p.dt_min) {
+            dtmin = p.dt_min;
+            nx.dt_min_hits += 1;
+        }
+        if (dtmin > p.dt_max) {
+            dtmin = p.dt_max;
+        }
+        dtb = dtmin;
+        nx.dt_base = (double)dtb;
+    }
+    dt = dtb;
+    t_next = next_event_time(c, in.t);
+    remain = t_next - in.t;
+    if (remain <= (double)dt) {
+        if (remain < 0) {
+            remain = 0;
+        }
+        dt = (R)remain;
+        hit = 1;
+    }
+    sv.dt_base = dtb;
+    if (dt <= (R)0) {
+        if (c.steps <= 0) {
+            nx.state = FCA_CTL_DONE; /* fca_run stops when t does not advance */
+        }
+        return;
+    }
+    sv.active = 1;
+    sv.hit = hit;
+    sv.dt = dt;
+    sv.F = hyeto_integral(c, in.t, (double)dt);
+    sv.rain_f = (R)sv.F;
+    sv.tmax_val = (R)(in.t + (double)dt);
+    sv.den = (R)(2 * p.K) * dt;
+    sv.t_new = hit ? t_next : in.t + (double)dt;
+    sv.need_cand = ((in.step + 1) % (c.U > 0 ? c.U : 1)) == 0;
+    nx.t = sv.t_new;
+    nx.step = in.step + 1;
+}
+
+/* Order-preserving key of a time-step candidate for an unsigned atomic min
+ * (NaN candidates are dropped by the caller, as the reference's '<' test
+ * drops them). */
+FCA_HD uint32_t cand_key(float v)
+{
+    uint32_t b;
+    __builtin_memcpy(&b, &v, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+FCA_HD float cand_unkey(uint32_t k)
+{
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float v;
+    __builtin_memcpy(&v, &b, 4);
+    return v;
+}
+FCA_HD unsigned long long cand_key(double v)
+{
+    unsigned long long b;
+    __builtin_memcpy(&b, &v, 8);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+FCA_HD double cand_unkey(unsigned long long k)
+{
+    unsigned long long b = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    double v;
+    __builtin_memcpy(&v, &b, 8);
+    return v;
+}
+
+template <class R>
+struct CandKey;
+template <>
+struct CandKey<float> {
+    typedef uint32_t type;
+};
+template <>
+struct CandKey<double> {
+    typedef unsigned long long type;
+};
+
+/* Stable step of cell (r, c) read straight from grid arrays: dt_max unless a
+ * wet LAND cell (model.c:123-137, physics.h:158-217). */
+template <class R>
+FCA_HD R cell_dt_at(const R *z, const R *h, const R *n, const uint8_t *mask, int r, int c,
+                    int nr, int nc, const PhysK<R> &p)
+{
+    const int dr[8] = {0, 0, 1, -1, 1, 1, -1, -1};
+    const int dc[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+    size_t a = (size_t)r * (size_t)nc + (size_t)c;
+    int k, nk = (p.K == 8) ? 8 : 4;
+    R la, smax = (R)0, sl;
+    if (mask[a] != FCA_LAND || h[a] < p.tau_h) {
+        return p.dt_max;
+    }
+    la = z[a] + h[a];
+    for (k = 0; k < nk; k++) {
+        int rb = r + dr[k], cb = c + dc[k];
+        size_t b;
+        if (rb < 0 || rb >= nr || cb < 0 || cb >= nc) {
+            continue;
+        }
+        b = (size_t)rb * (size_t)nc + (size_t)cb;
+        if (mask[b] == FCA_WALL) {
+            continue;
+        }
+        sl = abs_slope<R>(la, level_of<R>(mask[b], z[b], h[b], p), k < 4 ? p.dx : p.ddiag);
+        if (sl > smax) {
+            smax = sl;
+        }
+    }
+    if (p.open && (c == 0 || c == nc - 1 || r == 0 || r == nr - 1)) {
+        sl = abs_slope<R>(la, z[a] - p.S_bnd * p.dx, p.dx);
+        if (sl > smax) {
+            smax = sl;
+        }
+    }
+    return cell_dt_from_smax<R>(h[a], n[a], smax, p);
+}
+
+/* PhysK from the ABI's FcaPhys (physics.h:37-48 for the widths). */
+template <class R>
+inline PhysK<R> make_phys(const FcaPhys &f)
+{
+    PhysK<R> p;
+    p.dx = (R)f.dx, p.A = (R)f.A, p.g = (R)f.g, p.alpha = (R)f.alpha;
+    p.tau_h = (R)f.tau_h, p.tau_l = (R)f.tau_l, p.z_sea = (R)f.z_sea;
+    p.dt_max = (R)f.dt_max, p.dt_min = (R)f.dt_min, p.S_bnd = (R)f.S_bnd;
+    p.w = (f.K == 8) ? ((R)sqrt(2.0) - (R)1) * p.dx : p.dx;
+    p.ddiag = (R)sqrt(2.0) * p.dx;
+    p.U = f.U, p.K = f.K, p.open = f.boundary == FCA_BND_OPEN;
+    return p;
+}
+
+#endif /* __cplusplus */
+
+#endif

@@ -1,0 +1,757 @@
+/*
+ * fca_kernels.hip -- gfx950 kernels of the WCA step (one precision per build:
+ * real = float, or double with -DFCA_DOUBLE).
+ *
+ * FCA_HIP_FUSED: k_fused, one launch per step. Workgroup = 4 wavefronts, each
+ *   walking one strip of rows over 64 columns (fca_walk.h). The workgroup
+ *   first evaluates the step controller from the previous launch's record and
+ *   time-step candidate, so the time step never leaves the device; workgroup 0
+ *   writes the next record. Depth is double-buffered (ctrl.hbuf), hmax/tmax
+ *   are updated in place by the cell's owner, the balance terms are summed per
+ *   workgroup into a slot of its own (deterministic, no atomics), and the next
+ *   step's candidate is one atomic min per workgroup.
+ * FCA_HIP_SPLIT: the same step as controller / faces / limiter / update /
+ *   candidate launches over face arrays (the phase order of the reference
+ *   CUDA backend with the time step kept on the device). It shares only the
+ *   scalar formulas of fca_device.h with the fused path and serves as an
+ *   independent cross-check.
+ */
+#include <hip/hip_runtime.h>
+
+#include "fca_launch.h"
+#include "fca_walk.h"
+
+typedef real R;
+typedef CandKey<R>::type Key;
+
+#define FCA_BLOCK 256
+
+/* ---- wavefront shifts ---------------------------------------------------- */
+
+/* DPP wave_shl:1 / wave_shr:1: lane i reads lane i+1 / i-1; lanes with no
+ * source read 0. */
+__device__ __forceinline__ int lane_from_right(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int lane_from_left(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float lane_from_right(float x)
+{
+    return __int_as_float(lane_from_right(__float_as_int(x)));
+}
+__device__ __forceinline__ float lane_from_left(float x)
+{
+    return __int_as_float(lane_from_left(__float_as_int(x)));
+}
+__device__ __forceinline__ double lane_from_right(double x)
+{
+    long long b = __double_as_longlong(x);
+    int lo = lane_from_right((int)(unsigned)(b & 0xffffffffll));
+    int hi = lane_from_right((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double lane_from_left(double x)
+{
+    long long b = __double_as_longlong(x);
+    int lo = lane_from_left((int)(unsigned)(b & 0xffffffffll));
+    int hi = lane_from_left((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+/* Lane policy of the walk on the GPU: one work-item is one lane. */
+struct GpuLanes {
+    WalkLane<R> s;
+    int lane;
+    template <class F>
+    __device__ __forceinline__ void each(F f)
+    {
+        f(s, lane);
+    }
+    template <class T>
+    __device__ __forceinline__ void pull_right(T WalkLane<R>::*dst, T WalkLane<R>::*src)
+    {
+        s.*dst = lane_from_right(s.*src);
+    }
+    template <class T>
+    __device__ __forceinline__ void pull_left(T WalkLane<R>::*dst, T WalkLane<R>::*src)
+    {
+        s.*dst = lane_from_left(s.*src);
+    }
+};
+
+/* ---- reductions ----------------------------------------------------------- */
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off, 64);
+    }
+    return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_min(T v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        T o = __shfl_xor(v, off, 64);
+        v = (o < v) ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void atomic_min_key(uint32_t *p, uint32_t k) { atomicMin(p, k); }
+__device__ __forceinline__ void atomic_min_key(unsigned long long *p, unsigned long long k)
+{
+    atomicMin(p, k);
+}
+
+/* Block-wide min of a candidate, then one atomic min on the key. */
+__device__ void block_min_to(R v, Key *dst)
+{
+    __shared__ R sh[FCA_BLOCK / 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    v = wave_min(v);
+    if (lane == 0) {
+        sh[wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        R m = sh[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            m = (sh[w] < m) ? sh[w] : m;
+        }
+        if (m < (R)INFINITY) {
+            atomic_min_key(dst, cand_key(m));
+        }
+    }
+}
+
+/* ---- controller record ----------------------------------------------------- */
+
+struct CtlIO {
+    const FcaDevCtrl *in;
+    FcaDevCtrl *out;
+    const Key *cand_in;
+    Key *cand_out, *cand_reset;
+    FcaHipStepRec *rec;
+    int rec_cap;
+};
+
+/* Evaluates the controller; the writer (one work-item per launch) also stores
+ * the next record, the step log entry and the candidate-slot bookkeeping. */
+__device__ StepVals<R> ctl_step(const CtlIO &io, const FcaCtlParams &ctl, const PhysK<R> &p,
+                                int writer, int pingpong, int *hbuf)
+{
+    FcaDevCtrl in = *io.in, nx;
+    StepVals<R> sv;
+    ctrl_eval<R>(in, cand_unkey(*io.cand_in), ctl, p, sv, nx);
+    *hbuf = in.hbuf;
+    if (pingpong && sv.active) {
+        nx.hbuf = 1 - in.hbuf;
+    }
+    if (writer) {
+        if (sv.active) {
+            if (in.nrec < io.rec_cap) {
+                FcaHipStepRec r;
+                r.t = in.t;
+                r.F = sv.F;
+                r.dt_base = sv.dt_base;
+                r.dt = sv.dt;
+                r.step = in.step;
+                r.hit_event = sv.hit;
+                io.rec[in.nrec] = r;
+            }
+            nx.nrec = in.nrec + 1;
+        } else {
+            *io.cand_out = *io.cand_in; /* the state did not change */
+        }
+        *io.cand_reset = cand_key((R)INFINITY);
+        *io.out = nx;
+    }
+    return sv;
+}
+
+/* ---- fused step ---------------------------------------------------------- */
+
+struct FusedArgs {
+    const R *z, *n, *cr;
+    R *h0, *h1, *hmax, *tmax;
+    const uint8_t *mask;
+    int nr, nc, strip_rows, ngroups, ntiles;
+    PhysK<R> p;
+    FcaCtlParams ctl;
+    CtlIO io;
+    FcaAccSlot *slots;
+};
+
+template <int K8>
+__global__ void __launch_bounds__(FCA_BLOCK) k_fused(FusedArgs a)
+{
+    __shared__ StepVals<R> sh_sv;
+    __shared__ int sh_hbuf;
+    __shared__ double sh_acc[FCA_BLOCK / 64][4];
+    __shared__ long long sh_cap[FCA_BLOCK / 64];
+    __shared__ R sh_dtc[FCA_BLOCK / 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (FCA_BLOCK / 64) + wave;
+
+    if (threadIdx.x == 0) {
+        int hb;
+        sh_sv = ctl_step(a.io, a.ctl, a.p, blockIdx.x == 0, 1, &hb);
+        sh_hbuf = hb;
+    }
+    __syncthreads();
+    const StepVals<R> sv = sh_sv;
+    if (!sv.active) {
+        return;
+    }
+
+    GpuLanes lp;
+    lp.lane = lane;
+    if (tile < a.ntiles) {
+        WalkArgs<R> wa;
+        const int strip = tile / a.ngroups, grp = tile - strip * a.ngroups;
+        const int r0 = strip * a.strip_rows;
+        const int r1 = (r0 + a.strip_rows < a.nr) ? r0 + a.strip_rows : a.nr;
+        wa.z = a.z, wa.n = a.n, wa.cr = a.cr, wa.mask = a.mask;
+        wa.h_in = sh_hbuf ? a.h1 : a.h0;
+        wa.h_out = sh_hbuf ? a.h0 : a.h1;
+        wa.hmax = a.hmax, wa.tmax = a.tmax;
+        wa.nr = a.nr, wa.nc = a.nc, wa.strip_rows = a.strip_rows;
+        wa.p = a.p;
+        wa.sv = sv;
+        walk_strip<R, K8>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+    } else {
+        lp.s.v_rain = lp.s.clamp = lp.s.v_sea = lp.s.v_bnd = 0.0;
+        lp.s.cap = 0;
+        lp.s.dtc = (R)INFINITY;
+    }
+
+    {
+        double vr = wave_sum(lp.s.v_rain), cl = wave_sum(lp.s.clamp);
+        double vs = wave_sum(lp.s.v_sea), vb = wave_sum(lp.s.v_bnd);
+        long long cp = wave_sum(lp.s.cap);
+        R dm = wave_min(lp.s.dtc);
+        if (lane == 0) {
+            sh_acc[wave][0] = vr, sh_acc[wave][1] = cl, sh_acc[wave][2] = vs;
+            sh_acc[wave][3] = vb, sh_cap[wave] = cp, sh_dtc[wave] = dm;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        FcaAccSlot *sl = &a.slots[blockIdx.x];
+        double vr = 0, cl = 0, vs = 0, vb = 0;
+        long long cp = 0;
+        R dm = (R)INFINITY;
+        for (int w = 0; w < FCA_BLOCK / 64; w++) {
+            vr += sh_acc[w][0], cl += sh_acc[w][1], vs += sh_acc[w][2], vb += sh_acc[w][3];
+            cp += sh_cap[w];
+            dm = (sh_dtc[w] < dm) ? sh_dtc[w] : dm;
+        }
+        sl->v_rain += vr, sl->clamp += cl, sl->v_sea += vs, sl->v_bnd += vb, sl->cap += cp;
+        if (sv.need_cand && dm < (R)INFINITY) {
+            atomic_min_key(a.io.cand_out, cand_key(dm));
+        }
+    }
+}
+
+/* ---- split step ----------------------------------------------------------- */
+
+struct SplitArgs {
+    const R *z, *n, *cr;
+    R *h, *hmax, *tmax, *qx, *qy, *qd1, *qd2, *qo, *s;
+    const uint8_t *mask;
+    int nr, nc;
+    PhysK<R> p;
+    FcaCtlParams ctl;
+    CtlIO io;
+    StepVals<R> *sv;
+    FcaAccSlot *acc;
+};
+
+__global__ void k_split_ctrl(SplitArgs a)
+{
+    int hb;
+    *a.sv = ctl_step(a.io, a.ctl, a.p, 1, 0, &hb);
+}
+
+/* Raw face fluxes (model.c:393-482): thread (r, c) owns the faces indexed
+ * (r, c) in Qx, Qy, Qd1, Qd2 and the open slots of cell (r, c). */
+template <int K8>
+__global__ void __launch_bounds__(FCA_BLOCK) k_split_faces(SplitArgs a)
+{
+    const StepVals<R> sv = *a.sv;
+    const int nr = a.nr, nc = a.nc;
+    const size_t N = (size_t)nr * nc;
+    const size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x;
+    long long cap_n = 0;
+    if (!sv.active) {
+        return;
+    }
+    if (i < N) {
+        const int r = (int)(i / nc), c = (int)(i - (size_t)r * nc);
+        const PhysK<R> &p = a.p;
+        int cap;
+#define FACE(A_, B_, D_)                                                                    \
+    face_flux<R>(a.z[A_], a.h[A_], a.n[A_], a.mask[A_], a.z[B_], a.h[B_], a.n[B_], a.mask[B_], \
+                 (D_), sv.dt, sv.den, p, cap)
+        if (c + 1 < nc) {
+            a.qx[(size_t)r * (nc - 1) + c] = FACE(i, i + 1, p.dx);
+            cap_n += cap;
+        }
+        if (r + 1 < nr) {
+            a.qy[i] = FACE(i, i + nc, p.dx);
+            cap_n += cap;
+        }
+        if (K8 && r + 1 < nr && c + 1 < nc) {
+            a.qd1[(size_t)r * (nc - 1) + c] = FACE(i, i + nc + 1, p.ddiag);
+            cap_n += cap;
+            a.qd2[(size_t)r * (nc - 1) + c] = FACE(i + 1, i + nc, p.ddiag);
+            cap_n += cap;
+        }
+#undef FACE
+        if (p.open && a.mask[i] == FCA_LAND) {
+            const int on[4] = {c == 0, c == nc - 1, r == 0, r == nr - 1};
+            for (int k = 0; k < 4; k++) {
+                if (on[k]) {
+                    a.qo[4 * i + k] = open_flux<R>(a.z[i], a.h[i], a.n[i], sv.dt, sv.den, p, cap);
+                    cap_n += cap;
+                }
+            }
+        }
+    }
+    cap_n = wave_sum(cap_n);
+    if ((threadIdx.x & 63) == 0 && cap_n) {
+        atomicAdd((unsigned long long *)&a.acc->cap, (unsigned long long)cap_n);
+    }
+}
+
+/* Outgoing volume and limiter per cell (model.c:180-238, 488-502). */
+template <int K8>
+__global__ void __launch_bounds__(FCA_BLOCK) k_split_limiter(SplitArgs a)
+{
+    const StepVals<R> sv = *a.sv;
+    const int nr = a.nr, nc = a.nc;
+    const size_t N = (size_t)nr * nc;
+    const size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x;
+    if (!sv.active || i >= N) {
+        return;
+    }
+    const int r = (int)(i / nc), c = (int)(i - (size_t)r * nc);
+    const size_t w1 = (size_t)nc - 1;
+    R out = (R)0, q;
+    if (c + 1 < nc && (q = a.qx[r * w1 + c]) > (R)0) {
+        out += q;
+    }
+    if (c > 0 && (q = a.qx[r * w1 + c - 1]) < (R)0) {
+        out += -q;
+    }
+    if (r + 1 < nr && (q = a.qy[i]) > (R)0) {
+        out += q;
+    }
+    if (r > 0 && (q = a.qy[i - nc]) < (R)0) {
+        out += -q;
+    }
+    if (K8) {
+        if (r + 1 < nr && c + 1 < nc && (q = a.qd1[r * w1 + c]) > (R)0) {
+            out += q;
+        }
+        if (r > 0 && c > 0 && (q = a.qd1[(r - 1) * w1 + c - 1]) < (R)0) {
+            out += -q;
+        }
+        if (r + 1 < nr && c > 0 && (q = a.qd2[r * w1 + c - 1]) > (R)0) {
+            out += q;
+        }
+        if (r > 0 && c + 1 < nc && (q = a.qd2[(r - 1) * w1 + c]) < (R)0) {
+            out += -q;
+        }
+    }
+    out += a.qo[4 * i] + a.qo[4 * i + 1] + a.qo[4 * i + 2] + a.qo[4 * i + 3];
+    out = out * sv.dt;
+    a.s[i] = (a.mask[i] == FCA_LAND) ? limiter_s<R>(out, a.h[i], a.p.A) : (R)1;
+}
+
+/* Upwind-scaled face seen from cell i; `mine` = i is the face's a side. */
+__device__ __forceinline__ R scaled_face(R q, const R *s, size_t ia, size_t ib)
+{
+    return q * ((q >= (R)0) ? s[ia] : s[ib]);
+}
+
+/* Net inflow, rain, clamp and hmax/tmax per cell (model.c:241-319, 511-597). */
+template <int K8>
+__global__ void __launch_bounds__(FCA_BLOCK) k_split_update(SplitArgs a)
+{
+    const StepVals<R> sv = *a.sv;
+    const int nr = a.nr, nc = a.nc;
+    const size_t N = (size_t)nr * nc;
+    const size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x;
+    double vr = 0, cl = 0, vs = 0, vb = 0;
+    if (!sv.active) {
+        return;
+    }
+    if (i < N) {
+        const int r = (int)(i / nc), c = (int)(i - (size_t)r * nc);
+        const size_t w1 = (size_t)nc - 1;
+        const R dt = sv.dt, A = a.p.A;
+        const int land = a.mask[i] == FCA_LAND;
+        const R *s = a.s;
+        R net = (R)0, q, qos, su = s[i];
+        double sea = 0.0;
+#define SEA_OUT(nb_, cond_)                                              \
+    if (land && a.mask[nb_] == FCA_SEA && (cond_)) {                     \
+        sea += (double)dt * (double)((q > (R)0) ? q : -q);               \
+    }
+        if (c + 1 < nc) {
+            q = scaled_face(a.qx[r * w1 + c], s, i, i + 1);
+            net -= q;
+            SEA_OUT(i + 1, q > (R)0)
+        }
+        if (c > 0) {
+            q = scaled_face(a.qx[r * w1 + c - 1], s, i - 1, i);
+            net += q;
+            SEA_OUT(i - 1, q < (R)0)
+        }
+        if (r + 1 < nr) {
+            q = scaled_face(a.qy[i], s, i, i + nc);
+            net -= q;
+            SEA_OUT(i + nc, q > (R)0)
+        }
+        if (r > 0) {
+            q = scaled_face(a.qy[i - nc], s, i - nc, i);
+            net += q;
+            SEA_OUT(i - nc, q < (R)0)
+        }
+        if (K8) {
+            if (r + 1 < nr && c + 1 < nc) {
+                q = scaled_face(a.qd1[r * w1 + c], s, i, i + nc + 1);
+                net -= q;
+                SEA_OUT(i + nc + 1, q > (R)0)
+            }
+            if (r > 0 && c > 0) {
+                q = scaled_face(a.qd1[(r - 1) * w1 + c - 1], s, i - nc - 1, i);
+                net += q;
+                SEA_OUT(i - nc - 1, q < (R)0)
+            }
+            if (r + 1 < nr && c > 0) {
+                q = scaled_face(a.qd2[r * w1 + c - 1], s, i, i + nc - 1);
+                net -= q;
+                SEA_OUT(i + nc - 1, q > (R)0)
+            }
+            if (r > 0 && c + 1 < nc) {
+                q = scaled_face(a.qd2[(r - 1) * w1 + c], s, i - nc + 1, i);
+                net += q;
+                SEA_OUT(i - nc + 1, q < (R)0)
+            }
+        }
+#undef SEA_OUT
+        qos = a.qo[4 * i] * su + a.qo[4 * i + 1] * su + a.qo[4 * i + 2] * su +
+              a.qo[4 * i + 3] * su;
+        net -= qos;
+        vs = sea;
+        vb = (double)dt * (double)qos;
+        if (land) {
+            R rain = a.cr[i] * sv.rain_f;
+            R hn = a.h[i] + (dt * (net / A) + rain);
+            vr = (double)A * (double)rain;
+            if (hn < (R)0) {
+                cl = (double)(-hn) * (double)A;
+                hn = (R)0;
+            }
+            a.h[i] = hn;
+            if (hn > a.hmax[i]) {
+                a.hmax[i] = hn;
+                a.tmax[i] = sv.tmax_val;
+            }
+        }
+    }
+    vr = wave_sum(vr), cl = wave_sum(cl), vs = wave_sum(vs), vb = wave_sum(vb);
+    if ((threadIdx.x & 63) == 0) {
+        if (vr != 0.0) atomicAdd(&a.acc->v_rain, vr);
+        if (cl != 0.0) atomicAdd(&a.acc->clamp, cl);
+        if (vs != 0.0) atomicAdd(&a.acc->v_sea, vs);
+        if (vb != 0.0) atomicAdd(&a.acc->v_bnd, vb);
+    }
+}
+
+/* Next step's candidates from the updated state (model.c:111-156). */
+template <int K8>
+__global__ void __launch_bounds__(FCA_BLOCK) k_split_cand(SplitArgs a)
+{
+    const StepVals<R> sv = *a.sv;
+    const size_t N = (size_t)a.nr * a.nc;
+    const size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x;
+    R v = (R)INFINITY;
+    if (!sv.active || !sv.need_cand) {
+        return;
+    }
+    if (i < N) {
+        const int r = (int)(i / a.nc), c = (int)(i - (size_t)r * a.nc);
+        R d = cell_dt_at<R>(a.z, a.h, a.n, a.mask, r, c, a.nr, a.nc, a.p);
+        if (a.mask[i] == FCA_LAND && a.h[i] >= a.p.tau_h && d == d) {
+            v = d;
+        }
+    }
+    block_min_to(v, a.io.cand_out);
+}
+
+/* ---- utility kernels ------------------------------------------------------ */
+
+template <int K8>
+__global__ void __launch_bounds__(FCA_BLOCK)
+    k_cand_init(const R *z, const R *h, const R *n, const uint8_t *mask, int nr, int nc,
+                PhysK<R> p, Key *dst)
+{
+    const size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x;
+    R v = (R)INFINITY;
+    if (i < (size_t)nr * nc) {
+        const int r = (int)(i / nc), c = (int)(i - (size_t)r * nc);
+        R d = cell_dt_at<R>(z, h, n, mask, r, c, nr, nc, p);
+        if (mask[i] == FCA_LAND && h[i] >= p.tau_h && d == d) {
+            v = d;
+        }
+    }
+    block_min_to(v, dst);
+}
+
+__global__ void k_store_key(Key *dst, Key v) { *dst = v; }
+
+__global__ void __launch_bounds__(FCA_BLOCK) k_cr(const R *C, const R *Rday, R *cr, size_t N)
+{
+    for (size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * FCA_BLOCK) {
+        cr[i] = C[i] * Rday[i];
+    }
+}
+
+/* Fixed-partition partial sums, so the total is the same on every run. */
+__global__ void __launch_bounds__(FCA_BLOCK)
+    k_volume(const R *h, const uint8_t *mask, size_t N, R A, double *red)
+{
+    __shared__ double sh[FCA_BLOCK / 64];
+    double v = 0.0;
+    for (size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * FCA_BLOCK) {
+        if (mask[i] == FCA_LAND) {
+            v += (double)h[i] * (double)A;
+        }
+    }
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < FCA_BLOCK / 64; w++) {
+            t += sh[w];
+        }
+        red[blockIdx.x] = t;
+    }
+}
+
+#define FCA_MAXTHR 4
+struct Thr {
+    double v[FCA_MAXTHR];
+    int n;
+};
+
+/* Ensemble counts (ens_common.c:166-188): red holds per block
+ * [flooded_0..flooded_3, max LAND hmax (at least 0)]. */
+__global__ void __launch_bounds__(FCA_BLOCK)
+    k_ens_count(const R *hmax, const uint8_t *mask, size_t N, Thr thr, uint32_t *counts,
+                double *red)
+{
+    __shared__ double sh[FCA_BLOCK / 64][FCA_MAXTHR + 1];
+    double f[FCA_MAXTHR] = {0, 0, 0, 0}, mx = 0.0;
+    for (size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * FCA_BLOCK) {
+        if (mask[i] != FCA_LAND) {
+            continue;
+        }
+        double hm = (double)hmax[i];
+        mx = (hm > mx) ? hm : mx;
+        for (int j = 0; j < thr.n; j++) {
+            if (hm >= thr.v[j]) {
+                counts[(size_t)j * N + i] += 1u;
+                f[j] += 1.0;
+            }
+        }
+    }
+    for (int j = 0; j < FCA_MAXTHR; j++) {
+        f[j] = wave_sum(f[j]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(mx, off, 64);
+        mx = (o > mx) ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        for (int j = 0; j < FCA_MAXTHR; j++) {
+            sh[threadIdx.x >> 6][j] = f[j];
+        }
+        sh[threadIdx.x >> 6][FCA_MAXTHR] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 0; j <= FCA_MAXTHR; j++) {
+            double t = sh[0][j];
+            for (int w = 1; w < FCA_BLOCK / 64; w++) {
+                t = (j < FCA_MAXTHR) ? t + sh[w][j] : ((sh[w][j] > t) ? sh[w][j] : t);
+            }
+            red[(size_t)blockIdx.x * (FCA_MAXTHR + 1) + j] = t;
+        }
+    }
+}
+
+/* ---- launchers (C ABI) ----------------------------------------------------- */
+
+static inline unsigned blocks_for(size_t n)
+{
+    size_t b = (n + FCA_BLOCK - 1) / FCA_BLOCK;
+    return (unsigned)(b ? b : 1);
+}
+
+static inline Key *cand_slot(const FcaDev *d, long long k)
+{
+    return (Key *)d->cand + (int)(((k % 3) + 3) % 3);
+}
+
+static CtlIO ctl_io(const FcaDev *d, long long k)
+{
+    CtlIO io;
+    io.in = &d->ctrl[k & 1];
+    io.out = &d->ctrl[(k + 1) & 1];
+    io.cand_in = cand_slot(d, k);
+    io.cand_out = cand_slot(d, k + 1);
+    io.cand_reset = cand_slot(d, k + 2);
+    io.rec = d->rec;
+    io.rec_cap = d->rec_cap;
+    return io;
+}
+
+extern "C" void fca_k_geometry(FcaDev *d)
+{
+    if (d->strip_rows <= 0) {
+        d->strip_rows = 256;
+    }
+    d->ngroups = (d->nc + FCA_OUTC - 1) / FCA_OUTC;
+    d->nstrips = (d->nr + d->strip_rows - 1) / d->strip_rows;
+    d->ntiles = d->ngroups * d->nstrips;
+    d->nblocks = (d->ntiles + FCA_BLOCK / 64 - 1) / (FCA_BLOCK / 64);
+}
+
+extern "C" int fca_k_cand_reset(const FcaDev *d, int slot, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_store_key, dim3(1), dim3(1), 0, st, (Key *)d->cand + slot,
+                       cand_key((R)INFINITY));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fca_k_cand_init(const FcaDev *d, int hbuf, int slot, hipStream_t st)
+{
+    const size_t N = (size_t)d->nr * d->nc;
+    const PhysK<R> p = make_phys<R>(d->phys);
+    Key *dst = (Key *)d->cand + slot;
+    if (fca_k_cand_reset(d, slot, st) != 0) {
+        return -1;
+    }
+    if (d->K == 8) {
+        hipLaunchKernelGGL(k_cand_init<1>, dim3(blocks_for(N)), dim3(FCA_BLOCK), 0, st, d->z,
+                           d->h[hbuf], d->n, d->mask, d->nr, d->nc, p, dst);
+    } else {
+        hipLaunchKernelGGL(k_cand_init<0>, dim3(blocks_for(N)), dim3(FCA_BLOCK), 0, st, d->z,
+                           d->h[hbuf], d->n, d->mask, d->nr, d->nc, p, dst);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fca_k_step(const FcaDev *d, long long k, hipStream_t st)
+{
+    const PhysK<R> p = make_phys<R>(d->phys);
+    if (d->kernel == FCA_HIP_FUSED) {
+        FusedArgs a;
+        a.z = d->z, a.n = d->n, a.cr = d->cr, a.mask = d->mask;
+        a.h0 = d->h[0], a.h1 = d->h[1], a.hmax = d->hmax, a.tmax = d->tmax;
+        a.nr = d->nr, a.nc = d->nc, a.strip_rows = d->strip_rows;
+        a.ngroups = d->ngroups, a.ntiles = d->ntiles;
+        a.p = p;
+        a.ctl = d->ctl;
+        a.io = ctl_io(d, k);
+        a.slots = d->slots;
+        if (d->K == 8) {
+            hipLaunchKernelGGL(k_fused<1>, dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
+        } else {
+            hipLaunchKernelGGL(k_fused<0>, dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
+        }
+    } else {
+        SplitArgs a;
+        const size_t N = (size_t)d->nr * d->nc;
+        const dim3 g(blocks_for(N)), b(FCA_BLOCK);
+        a.z = d->z, a.n = d->n, a.cr = d->cr, a.mask = d->mask;
+        a.h = d->h[0], a.hmax = d->hmax, a.tmax = d->tmax;
+        a.qx = d->qx, a.qy = d->qy, a.qd1 = d->qd1, a.qd2 = d->qd2, a.qo = d->qo, a.s = d->s;
+        a.nr = d->nr, a.nc = d->nc;
+        a.p = p;
+        a.ctl = d->ctl;
+        a.io = ctl_io(d, k);
+        a.sv = (StepVals<R> *)d->sv;
+        a.acc = d->slots;
+        hipLaunchKernelGGL(k_split_ctrl, dim3(1), dim3(1), 0, st, a);
+        if (d->K == 8) {
+            hipLaunchKernelGGL(k_split_faces<1>, g, b, 0, st, a);
+            hipLaunchKernelGGL(k_split_limiter<1>, g, b, 0, st, a);
+            hipLaunchKernelGGL(k_split_update<1>, g, b, 0, st, a);
+            hipLaunchKernelGGL(k_split_cand<1>, g, b, 0, st, a);
+        } else {
+            hipLaunchKernelGGL(k_split_faces<0>, g, b, 0, st, a);
+            hipLaunchKernelGGL(k_split_limiter<0>, g, b, 0, st, a);
+            hipLaunchKernelGGL(k_split_update<0>, g, b, 0, st, a);
+            hipLaunchKernelGGL(k_split_cand<0>, g, b, 0, st, a);
+        }
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fca_k_cr(const FcaDev *d, hipStream_t st)
+{
+    const size_t N = (size_t)d->nr * d->nc;
+    unsigned g = blocks_for(N);
+    hipLaunchKernelGGL(k_cr, dim3(g < 65536 ? g : 65536), dim3(FCA_BLOCK), 0, st, d->C, d->Rday,
+                       d->cr, N);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fca_k_volume(const FcaDev *d, int hbuf, hipStream_t st)
+{
+    const size_t N = (size_t)d->nr * d->nc;
+    unsigned g = blocks_for(N);
+    if (g > (unsigned)d->nred) {
+        g = (unsigned)d->nred;
+    }
+    hipLaunchKernelGGL(k_volume, dim3(g), dim3(FCA_BLOCK), 0, st, d->h[hbuf], d->mask, N,
+                       (R)d->phys.A, d->red);
+    return hipGetLastError() == hipSuccess ? (int)g : -1;
+}
+
+extern "C" int fca_k_ens_count(const FcaDev *d, uint32_t *counts, int nthr,
+                               const double *thr_host, hipStream_t st)
+{
+    const size_t N = (size_t)d->nr * d->nc;
+    unsigned g = blocks_for(N);
+    Thr t;
+    if (nthr < 1 || nthr > FCA_MAXTHR) {
+        return -1;
+    }
+    for (int j = 0; j < FCA_MAXTHR; j++) {
+        t.v[j] = j < nthr ? thr_host[j] : 0.0;
+    }
+    t.n = nthr;
+    if (g > (unsigned)(d->nred / (FCA_MAXTHR + 1))) {
+        g = (unsigned)(d->nred / (FCA_MAXTHR + 1));
+    }
+    hipLaunchKernelGGL(k_ens_count, dim3(g), dim3(FCA_BLOCK), 0, st, d->hmax, d->mask, N, t,
+                       counts, d->red);
+    return hipGetLastError() == hipSuccess ? (int)g : -1;
+}

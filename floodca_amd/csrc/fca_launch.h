@@ -1,0 +1,61 @@
+/*
+ * fca_launch.h -- C-visible interface between the C host (hip_backend.c) and
+ * the HIP translation unit (fca_kernels.hip) of one precision.
+ */
+#ifndef FCA_LAUNCH_H
+#define FCA_LAUNCH_H
+
+#include "fca_device.h"
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device-resident state of one grid. */
+typedef struct FcaDev {
+    int nr, nc, K, kernel;
+    real *z, *n, *C, *Rday, *cr, *h[2], *hmax, *tmax;
+    uint8_t *mask;
+    /* FCA_HIP_SPLIT scratch: faces laid out as in fca_types.h:121-125 */
+    real *qx, *qy, *qd1, *qd2, *qo, *s;
+    void *sv;               /* split: device StepVals of the current step */
+    FcaDevCtrl *ctrl;       /* [2], indexed by launch parity */
+    void *cand;             /* [3] time-step candidate keys, indexed by launch % 3 */
+    FcaAccSlot *slots;      /* fused: one per workgroup; split: [0] */
+    int nslots;
+    FcaHipStepRec *rec;
+    int rec_cap;
+    FcaPhys phys;
+    FcaCtlParams ctl;
+    /* fused geometry */
+    int strip_rows, ngroups, nstrips, ntiles, nblocks;
+    double *red;            /* scratch of the volume / ensemble reductions */
+    int nred;
+} FcaDev;
+
+/* Fills the fused geometry of d from d->strip_rows. */
+void fca_k_geometry(FcaDev *d);
+/* Time-step candidates of the state in h[hbuf] into cand slot `slot`
+ * (resets the slot first). */
+int fca_k_cand_init(const FcaDev *d, int hbuf, int slot, hipStream_t st);
+/* One step launch with launch index k (ctrl parity, candidate rotation). */
+int fca_k_step(const FcaDev *d, long long k, hipStream_t st);
+/* cr = C*Rday on every cell. */
+int fca_k_cr(const FcaDev *d, hipStream_t st);
+/* Sets candidate slot `slot` to +inf. */
+int fca_k_cand_reset(const FcaDev *d, int slot, hipStream_t st);
+/* Per-block partial LAND volumes sum(h*A) of h[hbuf] into d->red[0..nblk). Returns nblk. */
+int fca_k_volume(const FcaDev *d, int hbuf, hipStream_t st);
+/* Ensemble count: counts[j*N+a] += (LAND && hmax >= thr[j]) (uint32, saturating
+ * uint16 copy kept alongside), plus per-block flooded counts and max LAND hmax into
+ * d->red. Returns the number of blocks. */
+int fca_k_ens_count(const FcaDev *d, uint32_t *counts, int nthr, const double *thr_host,
+                    hipStream_t st);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

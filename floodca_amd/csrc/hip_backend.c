@@ -1,0 +1,815 @@
+/*
+ * hip_backend.c -- C host of the MI355X backend: the entry points of
+ * include/floodca_hip.h. It owns device memory, uploads and downloads, the
+ * launch loop with its occasional controller read-back (multi-day forcing
+ * reloads, end of run), and the ensemble session. All arithmetic of the step
+ * runs in fca_kernels.hip.
+ *
+ * Built once per precision (real = float, or double with -DFCA_DOUBLE).
+ */
+#include "floodca_hip.h"
+#include "fca_launch.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+static __thread char g_err[512];
+
+const char *fca_hip_last_error(void)
+{
+    return g_err;
+}
+
+static int fail(int rc, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return rc;
+}
+
+#define HIP_OR(expr, rc)                                                                  \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            return fail((rc), "%s failed at %s:%d: %s", #expr, __FILE__, __LINE__,        \
+                        hipGetErrorString(e_));                                           \
+        }                                                                                 \
+    } while (0)
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ---- resident session ------------------------------------------------------ */
+
+struct FcaHipSim {
+    FcaDev d;
+    hipStream_t st;
+    size_t N;
+    long long k;           /* launches enqueued so far */
+    FcaAcc acc0;           /* accumulators at upload time */
+    long faces_per_step;   /* faces whose cap flag the reference counts per step */
+    int batch;             /* launches between controller read-backs */
+    FcaDevCtrl *h_ctrl;    /* pinned copy of a controller record */
+    double *h_red;         /* pinned copy of reduction partials */
+    real *h_stage;         /* pinned staging for forcing reloads */
+    FcaRunCfg *cfg;        /* copy of the run configuration */
+    double t_h2d, t_io;
+};
+
+static void dev_free(FcaDev *d)
+{
+    void *ptrs[] = {d->z, d->n, d->C, d->Rday, d->cr, d->h[0], d->h[1], d->hmax, d->tmax,
+                    d->mask, d->qx, d->qy, d->qd1, d->qd2, d->qo, d->s, d->sv, d->ctrl,
+                    d->cand, d->slots, d->rec, d->red};
+    size_t i;
+    for (i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) {
+        if (ptrs[i]) {
+            hipFree(ptrs[i]);
+        }
+    }
+    memset(d, 0, sizeof(*d));
+}
+
+void fca_hip_sim_close(FcaHipSim *s)
+{
+    if (!s) {
+        return;
+    }
+    if (s->st) {
+        hipStreamSynchronize(s->st);
+    }
+    dev_free(&s->d);
+    if (s->h_ctrl) {
+        hipHostFree(s->h_ctrl);
+    }
+    if (s->h_red) {
+        hipHostFree(s->h_red);
+    }
+    if (s->h_stage) {
+        hipHostFree(s->h_stage);
+    }
+    if (s->st) {
+        hipStreamDestroy(s->st);
+    }
+    free(s->cfg);
+    free(s);
+}
+
+static long count_faces(const FcaGrid *g, const FcaPhys *p)
+{
+    long nr = g->nrows, nc = g->ncols, f, r, c;
+    f = nr * (nc - 1) + (nr - 1) * nc + (p->K == 8 ? 2 * (nr - 1) * (nc - 1) : 0);
+    if (p->boundary == FCA_BND_OPEN) {
+        for (r = 0; r < nr; r++) {
+            for (c = 0; c < nc; c++) {
+                if (g->mask[r * nc + c] == FCA_LAND) {
+                    f += (c == 0) + (c == nc - 1) + (r == 0) + (r == nr - 1);
+                }
+            }
+        }
+    }
+    return f;
+}
+
+#define NRED 4096
+
+static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
+                     const FcaHipOpts *o)
+{
+    FcaDev *d = &s->d;
+    const size_t N = (size_t)g->nrows * (size_t)g->ncols, B = N * sizeof(real);
+    d->nr = g->nrows;
+    d->nc = g->ncols;
+    d->K = cfg->phys.K;
+    d->kernel = o ? o->kernel : FCA_HIP_FUSED;
+    d->strip_rows = (o && o->strip_rows > 0) ? o->strip_rows : 256;
+    d->rec_cap = (o && o->record_steps > 0) ? o->record_steps : 0;
+    d->phys = cfg->phys;
+    d->ctl.T_sim = (double)cfg->T_sim;
+    d->ctl.out_every = (double)cfg->out_every;
+    d->ctl.D_h = (double)cfg->D_h;
+    d->ctl.phi = (double)cfg->phi;
+    d->ctl.hyeto = cfg->hyeto;
+    d->ctl.steps = cfg->steps;
+    d->ctl.U = cfg->phys.U;
+    d->ctl.n_force_days = cfg->n_force_days;
+    fca_k_geometry(d);
+    s->N = N;
+    s->batch = (o && o->graph_batch > 0) ? o->graph_batch : 32;
+    HIP_OR(hipStreamCreate(&s->st), -2);
+    HIP_OR(hipMalloc((void **)&d->z, B), -2);
+    HIP_OR(hipMalloc((void **)&d->n, B), -2);
+    HIP_OR(hipMalloc((void **)&d->C, B), -2);
+    HIP_OR(hipMalloc((void **)&d->Rday, B), -2);
+    HIP_OR(hipMalloc((void **)&d->cr, B), -2);
+    HIP_OR(hipMalloc((void **)&d->h[0], B), -2);
+    HIP_OR(hipMalloc((void **)&d->h[1], B), -2);
+    HIP_OR(hipMalloc((void **)&d->hmax, B), -2);
+    HIP_OR(hipMalloc((void **)&d->tmax, B), -2);
+    HIP_OR(hipMalloc((void **)&d->mask, N), -2);
+    HIP_OR(hipMalloc((void **)&d->ctrl, 2 * sizeof(FcaDevCtrl)), -2);
+    HIP_OR(hipMalloc((void **)&d->cand, 3 * sizeof(double)), -2);
+    d->nslots = d->kernel == FCA_HIP_FUSED ? d->nblocks : 1;
+    HIP_OR(hipMalloc((void **)&d->slots, (size_t)d->nslots * sizeof(FcaAccSlot)), -2);
+    HIP_OR(hipMalloc((void **)&d->rec, (size_t)(d->rec_cap > 0 ? d->rec_cap : 1) *
+                                           sizeof(FcaHipStepRec)), -2);
+    d->nred = NRED * 5;
+    HIP_OR(hipMalloc((void **)&d->red, (size_t)d->nred * sizeof(double)), -2);
+    if (d->kernel == FCA_HIP_SPLIT) {
+        size_t nqx = (size_t)d->nr * (size_t)(d->nc - 1), nqy = (size_t)(d->nr - 1) * d->nc;
+        size_t nqd = (size_t)(d->nr - 1) * (size_t)(d->nc - 1);
+        HIP_OR(hipMalloc((void **)&d->qx, (nqx ? nqx : 1) * sizeof(real)), -2);
+        HIP_OR(hipMalloc((void **)&d->qy, (nqy ? nqy : 1) * sizeof(real)), -2);
+        HIP_OR(hipMalloc((void **)&d->qd1, (nqd ? nqd : 1) * sizeof(real)), -2);
+        HIP_OR(hipMalloc((void **)&d->qd2, (nqd ? nqd : 1) * sizeof(real)), -2);
+        HIP_OR(hipMalloc((void **)&d->qo, 4 * B), -2);
+        HIP_OR(hipMalloc((void **)&d->s, B), -2);
+        HIP_OR(hipMalloc((void **)&d->sv, 256), -2);
+        HIP_OR(hipMemset(d->qo, 0, 4 * B), -2);
+    }
+    HIP_OR(hipHostMalloc((void **)&s->h_ctrl, 2 * sizeof(FcaDevCtrl), hipHostMallocDefault), -2);
+    HIP_OR(hipHostMalloc((void **)&s->h_red, (size_t)d->nred * sizeof(double),
+                         hipHostMallocDefault), -2);
+    return 0;
+}
+
+static int sim_upload_static(FcaHipSim *s, const FcaGrid *g)
+{
+    FcaDev *d = &s->d;
+    const size_t B = s->N * sizeof(real);
+    double t0 = now_s();
+    HIP_OR(hipMemcpy(d->z, g->z, B, hipMemcpyHostToDevice), -2);
+    HIP_OR(hipMemcpy(d->n, g->n, B, hipMemcpyHostToDevice), -2);
+    HIP_OR(hipMemcpy(d->C, g->C, B, hipMemcpyHostToDevice), -2);
+    HIP_OR(hipMemcpy(d->mask, g->mask, s->N, hipMemcpyHostToDevice), -2);
+    s->t_h2d += now_s() - t0;
+    return 0;
+}
+
+/* Uploads the per-run state (h, hmax, tmax, Rday), resets the controller
+ * from acc and computes the first time-step candidate. */
+static int sim_upload_state(FcaHipSim *s, const FcaGrid *g, const FcaAcc *acc)
+{
+    FcaDev *d = &s->d;
+    const size_t B = s->N * sizeof(real);
+    FcaDevCtrl c;
+    double t0 = now_s();
+    HIP_OR(hipMemcpy(d->h[0], g->h, B, hipMemcpyHostToDevice), -2);
+    HIP_OR(hipMemcpy(d->h[1], d->h[0], B, hipMemcpyDeviceToDevice), -2);
+    HIP_OR(hipMemcpy(d->hmax, g->hmax, B, hipMemcpyHostToDevice), -2);
+    HIP_OR(hipMemcpy(d->tmax, g->tmax, B, hipMemcpyHostToDevice), -2);
+    HIP_OR(hipMemcpy(d->Rday, g->Rday, B, hipMemcpyHostToDevice), -2);
+    s->t_h2d += now_s() - t0;
+    memset(&c, 0, sizeof(c));
+    c.t = acc->t;
+    c.step = acc->step;
+    c.force_day = acc->force_day;
+    c.state = FCA_CTL_RUN;
+    c.hbuf = 0;
+    c.dt_base = (double)acc->dt_base;
+    HIP_OR(hipMemcpy(d->ctrl, &c, sizeof(c), hipMemcpyHostToDevice), -2);
+    HIP_OR(hipMemset(d->slots, 0, (size_t)d->nslots * sizeof(FcaAccSlot)), -2);
+    if (fca_k_cr(d, s->st) != 0 || fca_k_cand_init(d, 0, 0, s->st) != 0 ||
+        fca_k_cand_reset(d, 1, s->st) != 0 || fca_k_cand_reset(d, 2, s->st) != 0) {
+        return fail(-2, "kernel launch failed during upload");
+    }
+    HIP_OR(hipStreamSynchronize(s->st), -2);
+    s->k = 0;
+    s->acc0 = *acc;
+    return 0;
+}
+
+static int check_cfg(const FcaGrid *g, const FcaRunCfg *cfg, const FcaHipOpts *o)
+{
+    if (!g || !cfg || g->nrows <= 0 || g->ncols <= 0) {
+        return fail(-1, "empty grid");
+    }
+    if ((long long)g->nrows * (long long)g->ncols > (1ll << 31) - 1) {
+        return fail(-1, "grid of %d x %d cells exceeds 2^31-1 cells", g->nrows, g->ncols);
+    }
+    if (cfg->phys.K != 4 && cfg->phys.K != 8) {
+        return fail(-1, "K must be 4 or 8, not %d", cfg->phys.K);
+    }
+    if (o && o->kernel != FCA_HIP_FUSED && o->kernel != FCA_HIP_SPLIT) {
+        return fail(-1, "unknown kernel family %d", o->kernel);
+    }
+    if (o && o->strip_rows < 0) {
+        return fail(-1, "strip_rows must be >= 0");
+    }
+    if (cfg->n_force_days < 0 || cfg->n_force_days > FCA_MAX_FORCE_DAYS) {
+        return fail(-1, "n_force_days out of range");
+    }
+    return 0;
+}
+
+FcaHipSim *fca_hip_sim_open(const FcaGrid *g, const FcaRunCfg *cfg, const FcaAcc *acc,
+                            const FcaHipOpts *opts)
+{
+    FcaHipSim *s;
+    g_err[0] = 0;
+    if (check_cfg(g, cfg, opts) != 0 || !acc) {
+        if (!acc) {
+            fail(-1, "acc is NULL");
+        }
+        return NULL;
+    }
+    s = (FcaHipSim *)calloc(1, sizeof(*s));
+    if (!s) {
+        fail(-2, "out of host memory");
+        return NULL;
+    }
+    s->cfg = (FcaRunCfg *)malloc(sizeof(FcaRunCfg));
+    if (!s->cfg) {
+        free(s);
+        fail(-2, "out of host memory");
+        return NULL;
+    }
+    memcpy(s->cfg, cfg, sizeof(FcaRunCfg));
+    s->faces_per_step = count_faces(g, &cfg->phys);
+    if (sim_alloc(s, g, cfg, opts) != 0 || sim_upload_static(s, g) != 0 ||
+        sim_upload_state(s, g, acc) != 0) {
+        char keep[512];
+        memcpy(keep, g_err, sizeof(keep));
+        fca_hip_sim_close(s);
+        memcpy(g_err, keep, sizeof(keep));
+        return NULL;
+    }
+    return s;
+}
+
+void *fca_hip_sim_stream(FcaHipSim *s)
+{
+    return s ? (void *)s->st : NULL;
+}
+
+int fca_hip_sim_advance(FcaHipSim *s, int nsteps)
+{
+    int i;
+    if (!s || nsteps < 0) {
+        return fail(-1, "bad arguments");
+    }
+    for (i = 0; i < nsteps; i++) {
+        if (fca_k_step(&s->d, s->k, s->st) != 0) {
+            return fail(-2, "step launch failed: %s", hipGetErrorString(hipGetLastError()));
+        }
+        s->k++;
+    }
+    return nsteps;
+}
+
+int fca_hip_sim_sync(FcaHipSim *s)
+{
+    HIP_OR(hipStreamSynchronize(s->st), -2);
+    return 0;
+}
+
+/* Current controller record (the one the next launch reads). */
+static int read_ctrl(FcaHipSim *s, FcaDevCtrl *c)
+{
+    HIP_OR(hipMemcpyAsync(s->h_ctrl, &s->d.ctrl[s->k & 1], sizeof(FcaDevCtrl),
+                          hipMemcpyDeviceToHost, s->st), -2);
+    HIP_OR(hipStreamSynchronize(s->st), -2);
+    *c = *s->h_ctrl;
+    return 0;
+}
+
+static int write_ctrl(FcaHipSim *s, const FcaDevCtrl *c)
+{
+    *s->h_ctrl = *c;
+    HIP_OR(hipMemcpyAsync(&s->d.ctrl[s->k & 1], s->h_ctrl, sizeof(FcaDevCtrl),
+                          hipMemcpyHostToDevice, s->st), -2);
+    HIP_OR(hipStreamSynchronize(s->st), -2);
+    return 0;
+}
+
+/* Loads forcing day `day` (float32 raster) into Rday, scaled by lambda
+ * (model.c:338-358, grid.c:239-254). Returns 0, or -1 if unreadable. */
+static int reload_day(FcaHipSim *s, int day)
+{
+    FcaDev *d = &s->d;
+    const FcaRunCfg *cfg = s->cfg;
+    FILE *fp;
+    float *tmp;
+    size_t i, got;
+    double t0 = now_s();
+    if (!s->h_stage &&
+        hipHostMalloc((void **)&s->h_stage, s->N * sizeof(real), hipHostMallocDefault) !=
+            hipSuccess) {
+        return fail(-2, "pinned staging allocation failed");
+    }
+    fp = fopen(cfg->force_files[day], "rb");
+    if (!fp) {
+        return -1;
+    }
+    tmp = (float *)malloc(s->N * sizeof(float));
+    if (!tmp) {
+        fclose(fp);
+        return fail(-2, "out of host memory");
+    }
+    got = fread(tmp, sizeof(float), s->N, fp);
+    fclose(fp);
+    if (got != s->N) {
+        free(tmp);
+        return -1;
+    }
+    for (i = 0; i < s->N; i++) {
+        s->h_stage[i] = (real)tmp[i];
+        if (cfg->lambda != (real)1) {
+            s->h_stage[i] *= cfg->lambda;
+        }
+    }
+    free(tmp);
+    s->t_io += now_s() - t0;
+    t0 = now_s();
+    HIP_OR(hipMemcpyAsync(d->Rday, s->h_stage, s->N * sizeof(real), hipMemcpyHostToDevice,
+                          s->st), -2);
+    if (fca_k_cr(d, s->st) != 0) {
+        return fail(-2, "cr launch failed");
+    }
+    HIP_OR(hipStreamSynchronize(s->st), -2);
+    s->t_h2d += now_s() - t0;
+    return 0;
+}
+
+typedef struct RunHooks {
+    void (*poll)(void *);
+    void *pctx;
+    int poll_every;
+} RunHooks;
+
+/* fca_run's loop (model.c:607-643): launches in batches, reading the
+ * controller record back between batches to stop at T_sim, to reload the
+ * forcing day, or to stop when a fixed-step run cannot advance. */
+static int sim_run_hooked(FcaHipSim *s, const RunHooks *hk)
+{
+    const FcaRunCfg *cfg = s->cfg;
+    FcaDevCtrl c;
+    int last_poll;
+    if (read_ctrl(s, &c) != 0) {
+        return -2;
+    }
+    last_poll = c.step;
+    for (;;) {
+        int n = s->batch, before = c.step;
+        if (c.state == FCA_CTL_DONE ||
+            (cfg->steps <= 0 && c.t >= (double)cfg->T_sim - 1e-15)) {
+            break;
+        }
+        if (c.state == FCA_CTL_PAUSE) {
+            int rc = reload_day(s, c.want_day);
+            if (rc < -1) {
+                return rc;
+            }
+            if (rc == -1) { /* missing day file: the run ends at T_sim */
+                c.t = (double)cfg->T_sim;
+                c.state = FCA_CTL_DONE;
+            } else {
+                c.force_day = c.want_day;
+                c.state = FCA_CTL_RUN;
+            }
+            if (write_ctrl(s, &c) != 0) {
+                return -2;
+            }
+            continue;
+        }
+        if (cfg->steps > 0) {
+            if (c.step >= cfg->steps) {
+                break;
+            }
+            if (n > cfg->steps - c.step) {
+                n = cfg->steps - c.step;
+            }
+        }
+        if (fca_hip_sim_advance(s, n) < 0 || read_ctrl(s, &c) != 0) {
+            return -2;
+        }
+        if (hk && hk->poll && hk->poll_every > 0 &&
+            c.step / hk->poll_every != last_poll / hk->poll_every) {
+            hk->poll(hk->pctx);
+            last_poll = c.step;
+        }
+        if (cfg->progress_every > 0 &&
+            c.step / cfg->progress_every != before / cfg->progress_every) {
+            printf("progress step=%d t=%.1f dt_base=%.4g\n", c.step, c.t, c.dt_base);
+            fflush(stdout);
+        }
+        if (c.state == FCA_CTL_RUN && c.step == before && cfg->steps > 0) {
+            break; /* no step possible (the reference would spin here) */
+        }
+    }
+    return 0;
+}
+
+int fca_hip_sim_run(FcaHipSim *s)
+{
+    if (!s) {
+        return fail(-1, "NULL session");
+    }
+    return sim_run_hooked(s, NULL);
+}
+
+/* Sums the per-workgroup balance slots in a fixed order. */
+static int read_slots(FcaHipSim *s, double sums[4], long long *cap)
+{
+    FcaDev *d = &s->d;
+    FcaAccSlot *h;
+    int i;
+    h = (FcaAccSlot *)malloc((size_t)d->nslots * sizeof(FcaAccSlot));
+    if (!h) {
+        return fail(-2, "out of host memory");
+    }
+    if (hipMemcpy(h, d->slots, (size_t)d->nslots * sizeof(FcaAccSlot), hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+        free(h);
+        return fail(-2, "slot read-back failed");
+    }
+    sums[0] = sums[1] = sums[2] = sums[3] = 0.0;
+    *cap = 0;
+    for (i = 0; i < d->nslots; i++) {
+        sums[0] += h[i].v_rain;
+        sums[1] += h[i].clamp;
+        sums[2] += h[i].v_sea;
+        sums[3] += h[i].v_bnd;
+        *cap += h[i].cap;
+    }
+    free(h);
+    return 0;
+}
+
+static int device_volume(FcaHipSim *s, int hbuf, double *V)
+{
+    int nb = fca_k_volume(&s->d, hbuf, s->st), i;
+    if (nb < 0) {
+        return fail(-2, "volume launch failed");
+    }
+    HIP_OR(hipMemcpyAsync(s->h_red, s->d.red, (size_t)nb * sizeof(double),
+                          hipMemcpyDeviceToHost, s->st), -2);
+    HIP_OR(hipStreamSynchronize(s->st), -2);
+    *V = 0.0;
+    for (i = 0; i < nb; i++) {
+        *V += s->h_red[i];
+    }
+    return 0;
+}
+
+static int sim_acc(FcaHipSim *s, const FcaDevCtrl *c, FcaAcc *acc)
+{
+    double sums[4] = {0, 0, 0, 0}, V = 0.0;
+    long long cap = 0;
+    if (read_slots(s, sums, &cap) != 0 || device_volume(s, c->hbuf, &V) != 0) {
+        return -2;
+    }
+    *acc = s->acc0;
+    acc->V_rain += sums[0];
+    acc->clamp_err += sums[1];
+    acc->V_sea += sums[2];
+    acc->V_bnd += sums[3];
+    acc->cap_active += (long)cap;
+    acc->cap_total += s->faces_per_step * (long)(c->step - s->acc0.step);
+    acc->dt_min_hits += (long)c->dt_min_hits;
+    acc->t = c->t;
+    acc->step = c->step;
+    acc->dt_base = (real)c->dt_base;
+    acc->force_day = c->force_day;
+    acc->V = V;
+    return 0;
+}
+
+int fca_hip_sim_read(FcaHipSim *s, real *h, real *hmax, real *tmax, FcaAcc *acc)
+{
+    FcaDevCtrl c;
+    const size_t B = s->N * sizeof(real);
+    if (!s) {
+        return fail(-1, "NULL session");
+    }
+    if (read_ctrl(s, &c) != 0) {
+        return -2;
+    }
+    if (h) {
+        HIP_OR(hipMemcpy(h, s->d.h[c.hbuf], B, hipMemcpyDeviceToHost), -2);
+    }
+    if (hmax) {
+        HIP_OR(hipMemcpy(hmax, s->d.hmax, B, hipMemcpyDeviceToHost), -2);
+    }
+    if (tmax) {
+        HIP_OR(hipMemcpy(tmax, s->d.tmax, B, hipMemcpyDeviceToHost), -2);
+    }
+    if (acc && sim_acc(s, &c, acc) != 0) {
+        return -2;
+    }
+    return 0;
+}
+
+int fca_hip_sim_steplog(FcaHipSim *s, FcaHipStepRec *out, int cap)
+{
+    FcaDevCtrl c;
+    int n;
+    if (!s || (cap > 0 && !out)) {
+        return fail(-1, "bad arguments");
+    }
+    if (read_ctrl(s, &c) != 0) {
+        return -2;
+    }
+    n = c.nrec < s->d.rec_cap ? c.nrec : s->d.rec_cap;
+    if (n > cap) {
+        n = cap;
+    }
+    if (n > 0) {
+        HIP_OR(hipMemcpy(out, s->d.rec, (size_t)n * sizeof(FcaHipStepRec),
+                         hipMemcpyDeviceToHost), -2);
+    }
+    return n;
+}
+
+/* ---- entry points with a reference counterpart ---------------------------- */
+
+int fca_hip_run(FcaGrid *g, FcaRunCfg *cfg, FcaAcc *acc, FcaTimers *tm)
+{
+    FcaHipSim *s;
+    double t0;
+    int rc;
+    if (!cfg || !acc) {
+        return fail(-1, "NULL cfg or acc");
+    }
+    if (!cfg->reduce_impl[0]) {
+        snprintf(cfg->reduce_impl, sizeof(cfg->reduce_impl), "wave");
+    }
+    t0 = now_s();
+    s = fca_hip_sim_open(g, cfg, acc, NULL);
+    if (!s) {
+        return g_err[0] && strstr(g_err, "failed") ? -2 : -1;
+    }
+    if (tm) {
+        tm->t_init += now_s() - t0 - s->t_h2d;
+        tm->t_h2d = (tm->t_h2d > 0 ? tm->t_h2d : 0) + s->t_h2d;
+    }
+    s->t_h2d = 0;
+    t0 = now_s();
+    rc = fca_hip_sim_run(s);
+    if (rc == 0) {
+        rc = fca_hip_sim_sync(s);
+    }
+    if (tm) {
+        tm->wall_total += now_s() - t0;
+        tm->t_io += s->t_io;
+        tm->t_h2d += s->t_h2d;
+        tm->t_flux = tm->t_limiter = tm->t_update = tm->t_dt = -1;
+    }
+    if (rc == 0) {
+        t0 = now_s();
+        rc = fca_hip_sim_read(s, g->h, g->hmax, g->tmax, acc);
+        if (tm) {
+            tm->t_d2h = (tm->t_d2h > 0 ? tm->t_d2h : 0) + (now_s() - t0);
+        }
+    }
+    fca_hip_sim_close(s);
+    return rc;
+}
+
+void fca_hip_device_name(char *name, size_t n)
+{
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if (!name || n == 0) {
+        return;
+    }
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        snprintf(name, n, "unknown");
+        return;
+    }
+    snprintf(name, n, "%s", prop.name);
+}
+
+int fca_hip_select_device(int local_rank)
+{
+    int n = 0, dev;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        return fail(-1, "no HIP device");
+    }
+    dev = (local_rank < 0 ? -local_rank : local_rank) % n;
+    if (hipSetDevice(dev) != hipSuccess) {
+        return fail(-1, "hipSetDevice(%d) failed", dev);
+    }
+    return dev;
+}
+
+/* ---- ensemble session ------------------------------------------------------ */
+
+#define ENS_MAXTHR 4
+
+struct FcaHipEns {
+    FcaHipSim *s;
+    int nthr;
+    double thr[ENS_MAXTHR];
+    uint32_t *counts; /* device, nthr * N */
+};
+
+FcaHipEns *fca_hip_ens_open(const FcaGrid *g, FcaRunCfg *cfg, int nthr, const double *thr)
+{
+    FcaHipEns *e;
+    FcaAcc acc;
+    int j;
+    g_err[0] = 0;
+    if (nthr < 1 || nthr > ENS_MAXTHR || !thr) {
+        fail(-1, "fca_hip_ens_open: %d thresholds, at most %d", nthr, ENS_MAXTHR);
+        return NULL;
+    }
+    if (check_cfg(g, cfg, NULL) != 0) {
+        return NULL;
+    }
+    if (!cfg->reduce_impl[0]) {
+        snprintf(cfg->reduce_impl, sizeof(cfg->reduce_impl), "wave");
+    }
+    e = (FcaHipEns *)calloc(1, sizeof(*e));
+    if (!e) {
+        fail(-2, "out of host memory");
+        return NULL;
+    }
+    e->nthr = nthr;
+    for (j = 0; j < nthr; j++) {
+        e->thr[j] = thr[j];
+    }
+    memset(&acc, 0, sizeof(acc));
+    acc.force_day = -1;
+    e->s = fca_hip_sim_open(g, cfg, &acc, NULL);
+    if (!e->s) {
+        free(e);
+        return NULL;
+    }
+    if (hipMalloc((void **)&e->counts, (size_t)nthr * e->s->N * sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(e->counts, 0, (size_t)nthr * e->s->N * sizeof(uint32_t)) != hipSuccess) {
+        fail(-2, "count raster allocation failed");
+        fca_hip_ens_close(e);
+        return NULL;
+    }
+    return e;
+}
+
+int fca_hip_ens_run(FcaHipEns *e, FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc,
+                    FcaTimers *tm, long *flooded, double *max_depth, void (*poll)(void *),
+                    void *pctx, int poll_every)
+{
+    FcaHipSim *s;
+    RunHooks hk;
+    FcaDevCtrl c;
+    double t0;
+    int nb, b, j, rc;
+    if (!e || !g || !cfg || !acc) {
+        return fail(-1, "bad arguments");
+    }
+    s = e->s;
+    if (g->nrows != s->d.nr || g->ncols != s->d.nc || cfg->phys.K != s->d.K) {
+        return fail(-1, "scenario grid or K differs from the session's");
+    }
+    /* per-scenario configuration: forcing shape, horizon, step count */
+    memcpy(s->cfg, cfg, sizeof(FcaRunCfg));
+    s->d.phys = cfg->phys;
+    s->d.ctl.T_sim = (double)cfg->T_sim;
+    s->d.ctl.out_every = (double)cfg->out_every;
+    s->d.ctl.D_h = (double)cfg->D_h;
+    s->d.ctl.phi = (double)cfg->phi;
+    s->d.ctl.hyeto = cfg->hyeto;
+    s->d.ctl.steps = cfg->steps;
+    s->d.ctl.U = cfg->phys.U;
+    s->d.ctl.n_force_days = cfg->n_force_days;
+    s->t_h2d = s->t_io = 0;
+    if (sim_upload_state(s, g, acc) != 0) {
+        return -2;
+    }
+    if (tm) {
+        tm->t_h2d = (tm->t_h2d > 0 ? tm->t_h2d : 0) + s->t_h2d;
+    }
+    hk.poll = poll;
+    hk.pctx = pctx;
+    hk.poll_every = poll_every;
+    t0 = now_s();
+    rc = sim_run_hooked(s, &hk);
+    if (rc == 0) {
+        rc = fca_hip_sim_sync(s);
+    }
+    if (tm) {
+        tm->wall_total += now_s() - t0;
+        tm->t_flux = tm->t_limiter = tm->t_update = tm->t_dt = -1;
+    }
+    if (rc != 0 || read_ctrl(s, &c) != 0 || sim_acc(s, &c, acc) != 0) {
+        return -2;
+    }
+    t0 = now_s();
+    nb = fca_k_ens_count(&s->d, e->counts, e->nthr, e->thr, s->st);
+    if (nb < 0) {
+        return fail(-2, "count launch failed");
+    }
+    HIP_OR(hipMemcpyAsync(s->h_red, s->d.red, (size_t)nb * 5 * sizeof(double),
+                          hipMemcpyDeviceToHost, s->st), -2);
+    HIP_OR(hipStreamSynchronize(s->st), -2);
+    for (j = 0; j < e->nthr; j++) {
+        double f = 0.0;
+        for (b = 0; b < nb; b++) {
+            f += s->h_red[(size_t)b * 5 + j];
+        }
+        flooded[j] = (long)f;
+    }
+    *max_depth = 0.0;
+    for (b = 0; b < nb; b++) {
+        double m = s->h_red[(size_t)b * 5 + 4];
+        *max_depth = m > *max_depth ? m : *max_depth;
+    }
+    if (tm) {
+        tm->t_d2h = (tm->t_d2h > 0 ? tm->t_d2h : 0) + (now_s() - t0);
+    }
+    return 0;
+}
+
+int fca_hip_ens_counts_u32(FcaHipEns *e, uint32_t *host)
+{
+    if (!e || !host) {
+        return fail(-1, "bad arguments");
+    }
+    HIP_OR(hipMemcpy(host, e->counts, (size_t)e->nthr * e->s->N * sizeof(uint32_t),
+                     hipMemcpyDeviceToHost), -2);
+    return 0;
+}
+
+int fca_hip_ens_counts(FcaHipEns *e, uint16_t *host)
+{
+    size_t i, n;
+    uint32_t *tmp;
+    if (!e || !host) {
+        return fail(-1, "bad arguments");
+    }
+    n = (size_t)e->nthr * e->s->N;
+    tmp = (uint32_t *)malloc(n * sizeof(uint32_t));
+    if (!tmp) {
+        return fail(-2, "out of host memory");
+    }
+    if (fca_hip_ens_counts_u32(e, tmp) != 0) {
+        free(tmp);
+        return -2;
+    }
+    for (i = 0; i < n; i++) {
+        host[i] = (uint16_t)(tmp[i] > 65535u ? 65535u : tmp[i]);
+    }
+    free(tmp);
+    return 0;
+}
+
+void fca_hip_ens_close(FcaHipEns *e)
+{
+    if (!e) {
+        return;
+    }
+    if (e->counts) {
+        hipFree(e->counts);
+    }
+    fca_hip_sim_close(e->s);
+    free(e);
+}

@@ -1,0 +1,207 @@
+"""ctypes binding of the HIP backend (include/floodca_hip.h).
+
+Loads the in-tree ``libfloodca_hip.so`` (float) or ``libfloodca_hip_f64.so``
+(double). There is no fallback: if the library or a GPU is missing, calls
+raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import FCA_HIP_FUSED, abi
+from .build import lib_path
+
+_LIBS: dict[str, "Backend"] = {}
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class Backend:
+    """The C-ABI of one precision."""
+
+    def __init__(self, prec: str = "f32"):
+        path = lib_path(prec)
+        if not os.path.exists(path):
+            raise HipError(f"{path} is not built: run python -m floodca_amd.build")
+        self.prec = prec
+        self.a = a = abi(prec)
+        self.lib = L = C.CDLL(path)
+        G, Cf, Ac, Tm = (C.POINTER(a.FcaGrid), C.POINTER(a.FcaRunCfg), C.POINTER(a.FcaAcc),
+                         C.POINTER(a.FcaTimers))
+        P = C.POINTER(a.real)
+        sig = {
+            "fca_hip_run": (C.c_int, [G, Cf, Ac, Tm]),
+            "fca_hip_device_name": (None, [C.c_char_p, C.c_size_t]),
+            "fca_hip_select_device": (C.c_int, [C.c_int]),
+            "fca_hip_last_error": (C.c_char_p, []),
+            "fca_hip_sim_open": (C.c_void_p, [G, Cf, Ac, C.POINTER(a.FcaHipOpts)]),
+            "fca_hip_sim_advance": (C.c_int, [C.c_void_p, C.c_int]),
+            "fca_hip_sim_sync": (C.c_int, [C.c_void_p]),
+            "fca_hip_sim_read": (C.c_int, [C.c_void_p, P, P, P, Ac]),
+            "fca_hip_sim_steplog": (C.c_int, [C.c_void_p, C.POINTER(a.FcaHipStepRec), C.c_int]),
+            "fca_hip_sim_run": (C.c_int, [C.c_void_p]),
+            "fca_hip_sim_stream": (C.c_void_p, [C.c_void_p]),
+            "fca_hip_sim_close": (None, [C.c_void_p]),
+            "fca_hip_ens_open": (C.c_void_p, [G, Cf, C.c_int, C.POINTER(C.c_double)]),
+            "fca_hip_ens_run": (C.c_int, [C.c_void_p, G, Cf, Ac, Tm, C.POINTER(C.c_long),
+                                          C.POINTER(C.c_double), C.c_void_p, C.c_void_p,
+                                          C.c_int]),
+            "fca_hip_ens_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
+            "fca_hip_ens_counts_u32": (C.c_int, [C.c_void_p, C.c_void_p]),
+            "fca_hip_ens_close": (None, [C.c_void_p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+
+    def error(self) -> str:
+        return (self.lib.fca_hip_last_error() or b"").decode(errors="replace")
+
+    def check(self, rc, what):
+        if rc is None or (isinstance(rc, int) and rc < 0):
+            raise HipError(f"{what}: {self.error() or 'failed'} (rc={rc})")
+        return rc
+
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(128)
+        self.lib.fca_hip_device_name(buf, 128)
+        return buf.value.decode()
+
+    def run(self, grid, cfg, acc):
+        """fca_hip_run on grid in place; returns the FcaTimers."""
+        tm = self.a.FcaTimers()
+        tm.t_h2d = tm.t_d2h = tm.t_halo = tm.t_allreduce = tm.bytes_per_step = -1
+        g = grid.fca_grid(self.a)
+        self.check(self.lib.fca_hip_run(C.byref(g), C.byref(cfg), C.byref(acc), C.byref(tm)),
+                   "fca_hip_run")
+        return tm
+
+
+def backend(prec: str = "f32") -> Backend:
+    if prec not in _LIBS:
+        _LIBS[prec] = Backend(prec)
+    return _LIBS[prec]
+
+
+def opts(prec="f32", kernel=FCA_HIP_FUSED, strip_rows=0, graph_batch=0, record_steps=0):
+    o = abi(prec).FcaHipOpts()
+    o.kernel, o.strip_rows, o.graph_batch, o.record_steps = (kernel, strip_rows, graph_batch,
+                                                             record_steps)
+    return o
+
+
+class Sim:
+    """A grid resident in HBM (fca_hip_sim_*)."""
+
+    def __init__(self, grid, cfg, acc, prec="f32", **kw):
+        self.b = backend(prec)
+        self.a = self.b.a
+        self.grid, self.cfg = grid, cfg
+        self.opts = opts(prec, **kw)
+        g = grid.fca_grid(self.a)
+        self.h = self.b.lib.fca_hip_sim_open(C.byref(g), C.byref(cfg), C.byref(acc),
+                                             C.byref(self.opts))
+        if not self.h:
+            raise HipError(f"fca_hip_sim_open: {self.b.error()}")
+
+    def advance(self, n: int) -> int:
+        return self.b.check(self.b.lib.fca_hip_sim_advance(self.h, n), "advance")
+
+    def sync(self):
+        self.b.check(self.b.lib.fca_hip_sim_sync(self.h), "sync")
+
+    def run(self):
+        self.b.check(self.b.lib.fca_hip_sim_run(self.h), "run")
+
+    def read(self, grid=None):
+        """Copies h, hmax, tmax into grid (default: the grid given at open) and
+        returns the FcaAcc."""
+        grid = grid or self.grid
+        acc = self.a.FcaAcc()
+        P = C.POINTER(self.a.real)
+        self.b.check(self.b.lib.fca_hip_sim_read(self.h, grid.h.ctypes.data_as(P),
+                                                 grid.hmax.ctypes.data_as(P),
+                                                 grid.tmax.ctypes.data_as(P), C.byref(acc)),
+                     "read")
+        return acc
+
+    def steplog(self, cap: int):
+        buf = (self.a.FcaHipStepRec * max(cap, 1))()
+        n = self.b.check(self.b.lib.fca_hip_sim_steplog(self.h, buf, cap), "steplog")
+        return list(buf)[:n]
+
+    @property
+    def stream(self):
+        return self.b.lib.fca_hip_sim_stream(self.h)
+
+    def close(self):
+        if self.h:
+            self.b.lib.fca_hip_sim_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+POLL_FN = C.CFUNCTYPE(None, C.c_void_p)
+
+
+class Ensemble:
+    """fca_hip_ens_* session: static fields resident, one scenario per run()."""
+
+    def __init__(self, grid, cfg, thresholds=(0.10, 0.25, 0.50), prec="f32"):
+        self.b = backend(prec)
+        self.a = self.b.a
+        self.nthr = len(thresholds)
+        self.N = grid.nrows * grid.ncols
+        thr = (C.c_double * self.nthr)(*thresholds)
+        g = grid.fca_grid(self.a)
+        self.h = self.b.lib.fca_hip_ens_open(C.byref(g), C.byref(cfg), self.nthr, thr)
+        if not self.h:
+            raise HipError(f"fca_hip_ens_open: {self.b.error()}")
+
+    def run(self, grid, cfg, acc, poll=None, poll_every=0):
+        tm = self.a.FcaTimers()
+        fl = (C.c_long * self.nthr)()
+        md = C.c_double(0)
+        g = grid.fca_grid(self.a)
+        cb = POLL_FN(poll) if poll else None
+        self.b.check(self.b.lib.fca_hip_ens_run(self.h, C.byref(g), C.byref(cfg), C.byref(acc),
+                                                C.byref(tm), fl, C.byref(md),
+                                                C.cast(cb, C.c_void_p) if cb else None, None,
+                                                poll_every), "ens_run")
+        return list(fl), md.value, tm
+
+    def counts(self, u32=True):
+        if u32:
+            out = np.zeros(self.nthr * self.N, dtype=np.uint32)
+            self.b.check(self.b.lib.fca_hip_ens_counts_u32(self.h, out.ctypes.data), "counts")
+        else:
+            out = np.zeros(self.nthr * self.N, dtype=np.uint16)
+            self.b.check(self.b.lib.fca_hip_ens_counts(self.h, out.ctypes.data), "counts")
+        return out.reshape(self.nthr, -1)
+
+    def close(self):
+        if self.h:
+            self.b.lib.fca_hip_ens_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

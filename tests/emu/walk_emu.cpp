@@ -1,0 +1,220 @@
+/*
+ * walk_emu.cpp -- host emulation of the fused step, for the CPU test suite
+ * only (never shipped, never a fallback: the product path is the HIP library).
+ *
+ * It runs floodca_amd/csrc/fca_walk.h's walk_strip with 64 lanes in lockstep
+ * (pull_left/pull_right are array shifts with 0 at the wavefront edge, as the
+ * DPP shifts read 0), tile by tile in the order of the kernel's workgroups,
+ * with the kernel's controller (ctrl_eval) and candidate rules. The CPU
+ * suite checks it bit for bit against the oracle, which pins the walk's
+ * dataflow (halos, ring, edges, K=8 diagonals) before any GPU run.
+ */
+#include "fca_walk.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef real R;
+
+struct CpuLanes {
+    WalkLane<R> s[FCA_WAVE];
+    template <class F>
+    void each(F f)
+    {
+        for (int l = 0; l < FCA_WAVE; l++) {
+            f(s[l], l);
+        }
+    }
+    template <class T>
+    void pull_right(T WalkLane<R>::*dst, T WalkLane<R>::*src)
+    {
+        T tmp[FCA_WAVE];
+        for (int l = 0; l < FCA_WAVE; l++) {
+            tmp[l] = (l + 1 < FCA_WAVE) ? s[l + 1].*src : (T)0;
+        }
+        for (int l = 0; l < FCA_WAVE; l++) {
+            s[l].*dst = tmp[l];
+        }
+    }
+    template <class T>
+    void pull_left(T WalkLane<R>::*dst, T WalkLane<R>::*src)
+    {
+        T tmp[FCA_WAVE];
+        for (int l = 0; l < FCA_WAVE; l++) {
+            tmp[l] = (l > 0) ? s[l - 1].*src : (T)0;
+        }
+        for (int l = 0; l < FCA_WAVE; l++) {
+            s[l].*dst = tmp[l];
+        }
+    }
+};
+
+static int load_day(const char *path, R *dst, size_t n, R lambda)
+{
+    FILE *fp = fopen(path, "rb");
+    float *tmp;
+    size_t i, got;
+    if (!fp) {
+        return -1;
+    }
+    tmp = (float *)malloc(n * sizeof(float));
+    got = tmp ? fread(tmp, sizeof(float), n, fp) : 0;
+    fclose(fp);
+    if (got != n) {
+        free(tmp);
+        return -1;
+    }
+    for (i = 0; i < n; i++) {
+        dst[i] = (R)tmp[i];
+        if (lambda != (R)1) {
+            dst[i] *= lambda;
+        }
+    }
+    free(tmp);
+    return 0;
+}
+
+/* Runs the fused step like fca_hip_run, on the host. g is updated in place
+ * (h, hmax, tmax, Rday on reloads); acc as fca_hip_run sets it. */
+extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_rows,
+                       FcaHipStepRec *rec, int rec_cap, int *nrec)
+{
+    const int nr = g->nrows, nc = g->ncols, K8 = cfg->phys.K == 8;
+    const size_t N = (size_t)nr * (size_t)nc;
+    const PhysK<R> p = make_phys<R>(cfg->phys);
+    FcaCtlParams ctl;
+    FcaDevCtrl c, nx;
+    R *h[2], *cr, cand = (R)INFINITY;
+    long faces = 0;
+    int ngroups, nstrips, r, cc, hb = 0, nrc = 0;
+    size_t i;
+    const FcaAcc acc0 = *acc;
+
+    if (strip_rows <= 0) {
+        strip_rows = 256;
+    }
+    ctl.T_sim = (double)cfg->T_sim, ctl.out_every = (double)cfg->out_every;
+    ctl.D_h = (double)cfg->D_h, ctl.phi = (double)cfg->phi;
+    ctl.hyeto = cfg->hyeto, ctl.steps = cfg->steps, ctl.U = cfg->phys.U;
+    ctl.n_force_days = cfg->n_force_days;
+    h[0] = (R *)malloc(N * sizeof(R));
+    h[1] = (R *)malloc(N * sizeof(R));
+    cr = (R *)malloc(N * sizeof(R));
+    if (!h[0] || !h[1] || !cr) {
+        free(h[0]), free(h[1]), free(cr);
+        return -1;
+    }
+    memcpy(h[0], g->h, N * sizeof(R));
+    memcpy(h[1], g->h, N * sizeof(R));
+    for (i = 0; i < N; i++) {
+        cr[i] = g->C[i] * g->Rday[i];
+    }
+    for (r = 0; r < nr; r++) {
+        for (cc = 0; cc < nc; cc++) {
+            R d = cell_dt_at<R>(g->z, h[0], g->n, g->mask, r, cc, nr, nc, p);
+            size_t a = (size_t)r * nc + cc;
+            if (g->mask[a] == FCA_LAND && h[0][a] >= p.tau_h && d == d && d < cand) {
+                cand = d;
+            }
+            if (g->mask[a] == FCA_LAND && p.open) {
+                faces += (cc == 0) + (cc == nc - 1) + (r == 0) + (r == nr - 1);
+            }
+        }
+    }
+    faces += (long)nr * (nc - 1) + (long)(nr - 1) * nc + (K8 ? 2L * (nr - 1) * (nc - 1) : 0);
+    memset(&c, 0, sizeof(c));
+    c.t = acc->t, c.step = acc->step, c.force_day = acc->force_day;
+    c.state = FCA_CTL_RUN, c.dt_base = (double)acc->dt_base;
+    ngroups = (nc + FCA_OUTC - 1) / FCA_OUTC;
+    nstrips = (nr + strip_rows - 1) / strip_rows;
+
+    for (;;) {
+        StepVals<R> sv;
+        R next = (R)INFINITY;
+        if (c.state == FCA_CTL_DONE) {
+            break;
+        }
+        if (c.state == FCA_CTL_PAUSE) {
+            if (load_day(cfg->force_files[c.want_day], g->Rday, N, cfg->lambda) != 0) {
+                c.t = (double)cfg->T_sim;
+                c.state = FCA_CTL_DONE;
+                break;
+            }
+            for (i = 0; i < N; i++) {
+                cr[i] = g->C[i] * g->Rday[i];
+            }
+            c.force_day = c.want_day;
+            c.state = FCA_CTL_RUN;
+            continue;
+        }
+        if (cfg->steps > 0 && c.step >= cfg->steps) {
+            break;
+        }
+        ctrl_eval<R>(c, cand, ctl, p, sv, nx);
+        if (!sv.active) {
+            const int stuck = nx.state == FCA_CTL_RUN;
+            c = nx;
+            if (stuck) {
+                break; /* no step possible: the reference would spin */
+            }
+            continue;
+        }
+        if (rec && nrc < rec_cap) {
+            FcaHipStepRec q;
+            q.t = c.t, q.F = sv.F, q.dt_base = sv.dt_base, q.dt = sv.dt;
+            q.step = c.step, q.hit_event = sv.hit;
+            rec[nrc] = q;
+        }
+        nrc++;
+        {
+            WalkArgs<R> wa;
+            wa.z = g->z, wa.n = g->n, wa.cr = cr, wa.mask = g->mask;
+            wa.h_in = h[hb], wa.h_out = h[1 - hb], wa.hmax = g->hmax, wa.tmax = g->tmax;
+            wa.nr = nr, wa.nc = nc, wa.strip_rows = strip_rows, wa.p = p, wa.sv = sv;
+            for (int t = 0; t < ngroups * nstrips; t++) {
+                static CpuLanes lp;
+                const int strip = t / ngroups, grp = t % ngroups;
+                const int r0 = strip * strip_rows;
+                const int r1 = (r0 + strip_rows < nr) ? r0 + strip_rows : nr;
+                if (K8) {
+                    walk_strip<R, 1>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                } else {
+                    walk_strip<R, 0>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                }
+                for (int l = 0; l < FCA_WAVE; l++) {
+                    acc->V_rain += lp.s[l].v_rain;
+                    acc->clamp_err += lp.s[l].clamp;
+                    acc->V_sea += lp.s[l].v_sea;
+                    acc->V_bnd += lp.s[l].v_bnd;
+                    acc->cap_active += (long)lp.s[l].cap;
+                    if (lp.s[l].dtc < next) {
+                        next = lp.s[l].dtc;
+                    }
+                }
+            }
+        }
+        nx.hbuf = 1 - c.hbuf;
+        hb = nx.hbuf;
+        cand = sv.need_cand ? next : (R)INFINITY;
+        c = nx;
+    }
+    memcpy(g->h, h[hb], N * sizeof(R));
+    acc->cap_total = acc0.cap_total + faces * (long)(c.step - acc0.step);
+    acc->dt_min_hits = acc0.dt_min_hits + (long)c.dt_min_hits;
+    acc->t = c.t;
+    acc->step = c.step;
+    acc->dt_base = (R)c.dt_base;
+    acc->force_day = c.force_day;
+    acc->V = 0.0;
+    for (i = 0; i < N; i++) {
+        if (g->mask[i] == FCA_LAND) {
+            acc->V += (double)g->h[i] * (double)p.A;
+        }
+    }
+    if (nrec) {
+        *nrec = nrc;
+    }
+    free(h[0]), free(h[1]), free(cr);
+    return 0;
+}
