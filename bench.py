@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused WCA step on one MI355X.
+
+Workload (default): the single large grid of the plain method -- 16384^2
+cells at 90 m, seeded fractal DEM on a 0->60 m plane with a west sea (~70%
+LAND), 8-class Voronoi Manning n, C = 1, uniform 50 mm/h rain (Rday = 1.2 m,
+uniform hyetograph), dry start, K = 4, U = 1, FP32, open boundary. T_sim is
+set to 1e9 s so the run is bounded by the step count, as the reference's
+benchmarks do (--steps N --tsim 1e9).
+
+A "step" is one launch of the fused step kernel over the whole grid. W warmup
+steps run first (untimed; they also wet every LAND cell), then exactly K steps
+are timed with HIP events on the session's stream, inputs resident in HBM.
+
+Printed: one JSON line with value = LAND cell-updates/s (the reference's MCUPS
+definition x 1e6, src/common/log.c:133-135), the HBM roofline of the step
+kernel (algorithmic bytes per launch / average launch time vs 8 TB/s), and
+the reference serial build's throughput on the host cores (a bounded sample
+of the 512^2 case) as the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from floodca_amd import abi as A  # noqa: E402
+from floodca_amd import synth  # noqa: E402
+from floodca_amd.grid import write_dir  # noqa: E402
+from floodca_amd.hip import Sim, backend  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s & %HBM roofline; ensemble scenarios/h surrogate vs plain CA"
+
+
+def bytes_per_step(n_land: int, n_all: int, prec: str) -> int:
+    """Algorithmic HBM bytes of one fused step: each LAND cell reads z, n,
+    cr (= C*Rday), h, hmax and mask and writes h; every other cell reads only
+    its mask. The hmax/tmax writes of rising cells are not credited."""
+    w = 4 if prec == "f32" else 8
+    return n_land * (5 * w + 1 + w) + (n_all - n_land) * 1
+
+
+class HipTimer:
+    """HIP events on a given stream (the session's own)."""
+
+    def __init__(self):
+        try:
+            self.rt = C.CDLL("libamdhip64.so")
+        except OSError:
+            self.rt = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+        self.rt.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.rt.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.rt.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.rt.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.rt.hipEventDestroy.argtypes = [C.c_void_p]
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        for e in (self.a, self.b):
+            if self.rt.hipEventCreate(C.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+
+    def start(self, stream):
+        self.rt.hipEventRecord(self.a, stream)
+
+    def stop(self, stream) -> float:
+        self.rt.hipEventRecord(self.b, stream)
+        self.rt.hipEventSynchronize(self.b)
+        ms = C.c_float()
+        self.rt.hipEventElapsedTime(C.byref(ms), self.a, self.b)
+        return float(ms.value)
+
+    def device_sync(self):
+        self.rt.hipDeviceSynchronize()
+
+
+def load_traffic(workload: str):
+    """Per-launch HBM bytes measured with rocprofv3 --pmc for this workload,
+    as committed under profiles/ (None if not measured)."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p, encoding="utf-8") as fp:
+            return json.load(fp).get(workload)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(seconds: float, prec: str):
+    """The reference serial build (oracle/_ref, compiled from the reference's
+    sources with its stock -O3 flags) on the 512^2 case, one core, timed for
+    about `seconds` of CPU work. Falls back to the oracle restatement
+    ("port") when the reference build is absent."""
+    exe = os.path.join(ROOT, "oracle", "_ref", f"floodca_serial_{prec}")
+    g = synth.make_grid(512, 512, seed=20261203)
+    n_land = g.n_land
+    with tempfile.TemporaryDirectory() as td:
+        write_dir(td, g)
+        if os.path.exists(exe):
+            def run(steps):
+                out = subprocess.run([exe, "--grid", td, "--steps", str(steps), "--tsim", "1e9",
+                                      "--hyeto", "uniform", "--no-output", "--csv", ""],
+                                     capture_output=True, text=True, check=True).stdout
+                m = re.search(r"wall=([0-9.eE+-]+)", out)
+                return float(m.group(1))
+            w0 = run(20)
+            steps = max(20, int(seconds / max(w0 / 20, 1e-6)))
+            wall = run(steps)
+            kind = "reference"
+        else:
+            from oracle import Oracle
+            a = A.abi(prec)
+            o = Oracle(prec)
+            cfg = A.default_cfg(a)
+            A.set_dx(a, cfg, g.dx)
+            cfg.hyeto, cfg.T_sim = A.FCA_HYETO_UNIFORM, 1e9
+            gg = g.astype(a.dtype)
+            cfg.steps = 10
+            t = time.time()
+            o.run(gg.copy(), cfg)
+            w0 = time.time() - t
+            steps = max(10, int(seconds / max(w0 / 10, 1e-6)))
+            cfg.steps = steps
+            t = time.time()
+            o.run(gg, cfg)
+            wall = time.time() - t
+            kind = "port"
+    return {"value": n_land * steps / wall, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+            "sample": f"512^2 grid, {n_land} LAND cells, {steps} steps from a dry start with "
+                      f"50 mm/h rain, FP32, serial, wall {wall:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--size", type=int, default=16384)
+    ap.add_argument("--prec", choices=("f32", "f64"), default="f32")
+    ap.add_argument("--kernel", choices=("fused", "split"), default="fused")
+    ap.add_argument("--strip", type=int, default=0, help="rows per wavefront strip (0 = default)")
+    ap.add_argument("--neigh", type=int, default=4, choices=(4, 8))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(one grid fits one MI355X's 288 GB, and nothing here spans GPUs)",
+              file=sys.stderr)
+        return 2
+
+    prec = args.prec
+    a = A.abi(prec)
+    t0 = time.time()
+    g = synth.make_grid(args.size, args.size, seed=20261203, dtype=a.dtype)
+    t_gen = time.time() - t0
+    n_all, n_land = g.z.size, g.n_land
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, g.dx)
+    cfg.phys.K = cfg.neigh = args.neigh
+    cfg.hyeto = A.FCA_HYETO_UNIFORM
+    cfg.T_sim = 1e9
+    acc = a.FcaAcc()
+    acc.force_day = -1
+    acc.V0 = acc.V = 0.0
+    kern = A.FCA_HIP_FUSED if args.kernel == "fused" else A.FCA_HIP_SPLIT
+    print(f"[bench] grid {args.size}^2 generated in {t_gen:.1f}s, LAND {n_land}, "
+          f"device {backend(prec).device_name()}", file=sys.stderr, flush=True)
+
+    timer = HipTimer()
+    with Sim(g, cfg, acc, prec=prec, kernel=kern, strip_rows=args.strip) as sim:
+        stream = C.c_void_p(sim.stream)
+        sim.advance(args.warmup)
+        sim.sync()
+        timer.device_sync()
+        timer.start(stream)
+        sim.advance(args.steps)
+        ms = timer.stop(stream)
+        timer.device_sync()
+        out = sim.read()
+    steps_done = out.step
+    if steps_done != args.warmup + args.steps:
+        print(f"[bench] warning: {steps_done} steps taken of {args.warmup + args.steps}",
+              file=sys.stderr)
+    ms_per_step = ms / args.steps
+    value = n_land * args.steps / (ms * 1e-3)
+    bps = bytes_per_step(n_land, n_all, prec)
+    achieved = bps / (ms_per_step * 1e-3) / 1e9
+    workload = f"fused-{prec}-K{args.neigh}-{args.size}"
+    traffic = load_traffic(workload) if kern == A.FCA_HIP_FUSED else None
+    emass = abs(out.V + out.V_sea + out.V_bnd - out.V_rain - out.V0 - out.clamp_err) / \
+        max(out.V_rain + out.V0, 1e-30)
+    print(f"[bench] {ms_per_step:.4f} ms/step, {achieved:.0f} GB/s algorithmic, "
+          f"t={out.t:.0f}s, E_mass={emass:.2e}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": prec,
+        "data": "synthetic (seeded fractal DEM, Voronoi land cover, uniform rain)",
+        "config": {
+            "workload": f"plain WCA step, single grid {args.size}^2 at 90 m, {prec.upper()}, "
+                        f"K={args.neigh}, U=1, 50 mm/h uniform rain from a dry start, "
+                        f"{args.kernel} kernel",
+            "grid": [args.size, args.size], "land_cells": n_land, "all_cells": n_all,
+            "kernel": args.kernel, "neigh": args.neigh, "sim_time_s": out.t,
+            "all_cell_updates_per_s": n_all * args.steps / (ms * 1e-3),
+            "mass_err_rel": emass,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_launch": bps,
+            "bytes_rule": "per LAND cell z,n,cr,h,hmax + mask read, h written; 1 B mask per other cell",
+        },
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds, prec)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -37,7 +37,9 @@ PRECS = {"f32": ([], "libfloodca_hip.so"), "f64": (["-DFCA_DOUBLE"], "libfloodca
 
 
 def lib_path(prec: str = "f32") -> str:
-    return os.path.join(PKG, PRECS[prec][1])
+    """The in-tree backend library; FCA_HIP_LIB (float) / FCA_HIP_LIB_F64 override it."""
+    env = os.environ.get("FCA_HIP_LIB" if prec == "f32" else "FCA_HIP_LIB_F64")
+    return env or os.path.join(PKG, PRECS[prec][1])
 
 
 def _run(cmd, quiet=False):
@@ -51,16 +53,16 @@ def _run(cmd, quiet=False):
     return out
 
 
-def build_hip(prec: str, extra=()) -> str:
+def build_hip(prec: str, extra=(), out: str | None = None, tag: str = "") -> str:
     os.makedirs(BUILD, exist_ok=True)
     defs, name = PRECS[prec]
-    kobj = os.path.join(BUILD, f"fca_kernels_{prec}.o")
-    cobj = os.path.join(BUILD, f"hip_backend_{prec}.o")
+    kobj = os.path.join(BUILD, f"fca_kernels_{prec}{tag}.o")
+    cobj = os.path.join(BUILD, f"hip_backend_{prec}{tag}.o")
     hipcc = os.path.join(ROCM, "bin", "hipcc")
     _run([hipcc, *HIP_FLAGS, *defs, *extra, "-c", os.path.join(CSRC, "fca_kernels.hip"),
           "-o", kobj])
     _run(["gcc", *C_FLAGS, *defs, "-c", os.path.join(CSRC, "hip_backend.c"), "-o", cobj])
-    out = lib_path(prec)
+    out = out or os.path.join(PKG, name)
     tmp = out + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", kobj, cobj, "-o", tmp])
     os.replace(tmp, out)
@@ -75,6 +77,17 @@ def build_emu(prec: str) -> str:
     return out
 
 
+def build_mathcheck() -> str:
+    """Test-only HIP library evaluating the device scalar math (tests/gpu)."""
+    out = os.path.join(ROOT, "tests", "gpu", "libmathcheck.so")
+    hipcc = os.path.join(ROCM, "bin", "hipcc")
+    tmp = out + ".tmp"
+    _run([hipcc, *HIP_FLAGS, "-shared", os.path.join(ROOT, "tests", "gpu", "mathcheck.hip"),
+          "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle() -> None:
     _run(["bash", os.path.join(ROOT, "oracle", "build_oracle.sh")])
 
@@ -84,7 +97,8 @@ def build_all(verbose: bool = True) -> list[str]:
         raise RuntimeError(f"hipcc not found under {ROCM}")
     jobs = [(build_hip, "f32"), (build_hip, "f64"), (build_emu, "f32"), (build_emu, "f64")]
     with ThreadPoolExecutor(max_workers=4) as ex:
-        futs = [ex.submit(f, p) for f, p in jobs] + [ex.submit(build_oracle)]
+        futs = [ex.submit(f, p) for f, p in jobs] + [ex.submit(build_oracle),
+                                                     ex.submit(build_mathcheck)]
         outs = [f.result() for f in futs]
     if verbose:
         for o in outs:

@@ -61,13 +61,154 @@ struct PhysK {
     R w;     /* face width: dx (K=4) or (sqrt2-1)*dx on every face (K=8) */
     R ddiag; /* sqrt2*dx */
     int U, K, open;
+    /* Division by the run constants dx, sqrt2*dx and A: reciprocals and, per
+     * divisor, whether q0 = a*r, q = fma(fma(-q0, d, a), r, q0) was verified
+     * on the host to equal a/d for every float mantissa (div_const_exact). */
+    R rdx, rddiag, rA;
+    int fdx, fddiag, fA;
 };
 
-/* hf^(2/3) as (real)cbrt((double)hf*(double)hf) (physics.h:19-24). */
+/* a/d for a run-constant d with reciprocal rd = RN(1/d). With ok set (the
+ * two-fma form was verified exact for every mantissa of a), the device uses
+ * it inside the range where the scaling argument holds; otherwise, and on the
+ * host, a plain correctly rounded division. */
+FCA_HD float div_c(float a, float d, float rd, int ok)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float aa = __builtin_fabsf(a);
+    if (ok && aa >= 0x1p-80f && aa <= 0x1p80f) {
+        const float q0 = a * rd;
+        return __builtin_fmaf(__builtin_fmaf(-q0, d, a), rd, q0);
+    }
+#else
+    (void)rd, (void)ok;
+#endif
+    return a / d;
+}
+FCA_HD double div_c(double a, double d, double rd, int ok)
+{
+    (void)rd, (void)ok;
+    return a / d;
+}
+
+/* Correctly rounded 1/b: on the device a hardware reciprocal with one Newton
+ * correction, exact for every float in [2^-100, 2^100] (checked exhaustively
+ * over all mantissas on the GPU, tests/test_gpu_math.py). */
+FCA_HD float rcp_cr(float b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float ab = __builtin_fabsf(b);
+    if (ab >= 0x1p-100f && ab <= 0x1p100f) {
+        const float y = __builtin_amdgcn_rcpf(b);
+        return __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+    }
+#endif
+    return 1.0f / b;
+}
+FCA_HD double rcp_cr(double b) { return 1.0 / b; }
+
+/* Rough sqrt (relative error < 2^-21) used only to skip exact work. */
+FCA_HD float sqrt_rough(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+FCA_HD double sqrt_rough(double x) { return sqrt(x); }
+
+#define FCA_FILTER_MARGIN ((R)0x1p-18)
+
+/* ---- float-mode libm detours --------------------------------------------
+ *
+ * In float mode the reference evaluates hf^(2/3) as (float)cbrt((double)hf^2)
+ * and the Manning slope term as (float)sqrt((double)S) (physics.h:19-24, 106,
+ * 205). Both equal the correctly rounded float result for every float input:
+ * sqrt by the double-rounding theorem (53 >= 2*24+2), cbrt by an exhaustive
+ * check of all 2^31-2^23 positive finite floats against glibc 2.35
+ * (tests/test_exhaustive_libm.py). The device therefore computes correctly
+ * rounded float results from cheap hardware approximations:
+ *   cbrt_sq: v_log/v_exp seed, one Newton step in double, and a certificate
+ *     that the double result is far enough from a float rounding boundary;
+ *     otherwise an exact midpoint test (m^3 vs hf^2 in double-double);
+ *   sqrt: the compiler's correctly rounded sqrtf.
+ * The host side (the CPU emulation) keeps the reference's double formulas;
+ * the GPU test suite checks the device functions against numpy's libm over
+ * every positive float. */
+
+/* sign of m^3 - x, exact for m with <= 26 significant bits and x a double */
+FCA_HD int cube_cmp(double m, double x)
+{
+    double m2 = m * m;
+    double p = m2 * m;
+    double e = __builtin_fma(m2, m, -p);
+    double d = (p - x) + e;
+    return (d > 0.0) - (d < 0.0);
+}
+
+FCA_HD float f32_step(float f, int k)
+{
+    uint32_t b;
+    __builtin_memcpy(&b, &f, 4);
+    b += (uint32_t)k;
+    __builtin_memcpy(&f, &b, 4);
+    return f;
+}
+
+/* Correctly rounded hf^(2/3) from a float estimate y within one ulp. */
+FCA_HD float cbrt_sq_fix(float y, double x)
+{
+    float yn = f32_step(y, 1), yp = f32_step(y, -1);
+    if (cube_cmp(((double)y + (double)yn) * 0.5, x) < 0) {
+        return yn;
+    }
+    if (cube_cmp(((double)yp + (double)y) * 0.5, x) > 0) {
+        return yp;
+    }
+    return y;
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+/* Fast-path domain of cbrt_sq: 2^-15 <= hf <= 2^15 keeps |log2 hf| < 16, so
+ * the v_log/v_exp seed is within ~2^-20 and one Newton step lands within
+ * ~2^-41 of hf^(2/3). */
+#define FCA_CBRT_LO 0x1p-15f
+#define FCA_CBRT_HI 0x1p15f
+/* Reject when the double result is within 2^14 double ulps (2^-38 relative)
+ * of a float rounding midpoint. */
+#define FCA_CBRT_GUARD 16384
+
+/* One Newton step from the hardware seed: a^(2/3) to ~2^-41 relative. */
+__device__ inline double cbrt_sq_newton(float a, double sq)
+{
+    const float y0 = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(a) * (2.0f / 3.0f));
+    const double yd = (double)y0;
+    const double r = __builtin_fma(-(yd * yd), yd, sq); /* a^2 - y0^3, one rounding */
+    return yd + r * (double)__builtin_amdgcn_rcpf(3.0f * y0 * y0);
+}
+#endif
+
 FCA_HD float cbrt_sq(float hf)
 {
     double sq = (double)hf * (double)hf;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float a = __builtin_fabsf(hf);
+    if (a >= FCA_CBRT_LO && a <= FCA_CBRT_HI) {
+        const double y1 = cbrt_sq_newton(a, sq);
+        uint64_t bits;
+        __builtin_memcpy(&bits, &y1, 8);
+        /* the 29 bits below float precision, relative to the midpoint */
+        const int lo = (int)((uint32_t)bits & 0x1FFFFFFFu) - 0x10000000;
+        if (lo > FCA_CBRT_GUARD || lo < -FCA_CBRT_GUARD) {
+            return (float)y1;
+        }
+        return cbrt_sq_fix((float)y1, sq);
+    }
+    return cbrt_sq_fix((float)cbrt(sq), sq);
+#else
     return (float)cbrt(sq);
+#endif
 }
 FCA_HD double cbrt_sq(double hf)
 {
@@ -75,80 +216,86 @@ FCA_HD double cbrt_sq(double hf)
     return cbrt(sq);
 }
 
-/* (real)sqrt((double)x): the double detour the reference takes in float mode
- * (physics.h:106,205). */
-FCA_HD float sqrt_via_double(float x) { return (float)sqrt((double)x); }
+FCA_HD float sqrt_via_double(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_sqrtf(x);
+#else
+    return (float)sqrt((double)x);
+#endif
+}
 FCA_HD double sqrt_via_double(double x) { return sqrt(x); }
 
 /* (real)sqrt((double)g*(double)h), the critical velocity (physics.h:108,207). */
 FCA_HD float crit_speed(float g, float h) { return (float)sqrt((double)g * (double)h); }
 FCA_HD double crit_speed(double g, double h) { return sqrt(g * h); }
 
-/* Face flux a->b, positive = a to b (physics.h:51-125). den = (real)(2K)*dt,
- * evaluated once per step by the caller exactly as the reference evaluates it
- * per face. */
+/* Face flux a->b, positive = a to b (physics.h:51-125). den = (real)(2K)*dt
+ * and rden = RN(1/den) are evaluated once per step by the caller; d is dx or
+ * sqrt2*dx, rd its reciprocal and dok its div_c flag. Two filters skip exact
+ * work whose outcome is already decided: the critical-velocity cap is only
+ * evaluated exactly when v is within 2^-18 of a rough sqrt(g*hf), and the
+ * level-equalisation cap only when a rough (|dL|*A)*rden is not clearly above
+ * Qmag; the rough values are within 2^-21 of the exact ones, so a skipped
+ * test could not have changed the result. */
 template <class R>
-FCA_HD R face_flux(R za, R ha, R na, int ma, R zb, R hb, R nb, int mb, R d, R dt, R den,
-                   const PhysK<R> &p, int &cap)
+FCA_HD R face_flux(R za, R ha, R na, int ma, R zb, R hb, R nb, int mb, R d, R rd, int dok,
+                   R dt, R den, R rden, const PhysK<R> &p, int &cap)
 {
-    R bed_a, bed_b, lev_a, lev_b, dL, adL, hf, nf, v, vcr, qmag;
+    /* Cheap tests first, all evaluated; the reference's early returns
+     * (walls, dead band, dry face, upwind sea) become one `dead` predicate
+     * and the expensive part runs under a single branch. Each expression is
+     * the reference's own, so a live face's value is unchanged. */
+    const int sea_a = ma == FCA_SEA, sea_b = mb == FCA_SEA;
+    const R bed_a = sea_a ? p.z_sea : za;
+    const R bed_b = sea_b ? p.z_sea : zb;
+    const R lev_a = bed_a + (sea_a ? (R)0 : ha);
+    const R lev_b = bed_b + (sea_b ? (R)0 : hb);
+    const R dL = lev_a - lev_b;
+    const R adL = (dL < (R)0) ? -dL : dL;
+    const R hf = ((lev_a > lev_b) ? lev_a : lev_b) - ((bed_a > bed_b) ? bed_a : bed_b);
+    const int up_sea = (dL > (R)0) ? sea_a : sea_b;
+    const int dead = (ma == FCA_WALL) | (mb == FCA_WALL) | (adL < p.tau_l) | (hf < p.tau_h) |
+                     up_sea;
+    R qmag = (R)0;
     cap = 0;
-    if (ma == FCA_WALL || mb == FCA_WALL) {
-        return (R)0;
-    }
-    bed_a = (ma == FCA_SEA) ? p.z_sea : za;
-    bed_b = (mb == FCA_SEA) ? p.z_sea : zb;
-    lev_a = bed_a + ((ma == FCA_SEA) ? (R)0 : ha);
-    lev_b = bed_b + ((mb == FCA_SEA) ? (R)0 : hb);
-    dL = lev_a - lev_b;
-    if (dL < (R)0) {
-        if (-dL < p.tau_l) {
-            return (R)0;
+    if (!dead) {
+        R nf = sea_a ? nb : (sea_b ? na : (R)0.5 * (na + nb));
+        R v;
+        if (nf < (R)1e-12) {
+            nf = (R)1e-12;
         }
-    } else if (dL < p.tau_l) {
-        return (R)0;
-    }
-    hf = ((lev_a > lev_b) ? lev_a : lev_b) - ((bed_a > bed_b) ? bed_a : bed_b);
-    if (hf < p.tau_h) {
-        return (R)0;
-    }
-    if (((dL > (R)0) ? ma : mb) == FCA_SEA) {
-        return (R)0;
-    }
-    if (ma == FCA_SEA) {
-        nf = nb;
-    } else if (mb == FCA_SEA) {
-        nf = na;
-    } else {
-        nf = (R)0.5 * (na + nb);
-    }
-    if (nf < (R)1e-12) {
-        nf = (R)1e-12;
-    }
-    adL = (dL < (R)0) ? -dL : dL;
-    v = ((R)1 / nf) * cbrt_sq(hf) * sqrt_via_double(adL / d);
-    vcr = crit_speed(p.g, hf);
-    if (v > vcr) {
-        v = vcr;
-    }
-    qmag = v * hf * p.w;
-    if (dt > (R)0 && p.K > 0) {
-        R qcap = adL * p.A / den;
-        if (qcap < qmag) {
-            cap = 1;
-            qmag = qcap;
+        v = rcp_cr(nf) * cbrt_sq(hf) * sqrt_via_double(div_c(adL, d, rd, dok));
+        if (!(v < sqrt_rough(p.g * hf) * ((R)1 - FCA_FILTER_MARGIN))) {
+            const R vcr = crit_speed(p.g, hf);
+            if (v > vcr) {
+                v = vcr;
+            }
         }
+        qmag = v * hf * p.w;
+        if (dt > (R)0 && p.K > 0) {
+            const R num = adL * p.A;
+            if (!(num * rden > qmag * ((R)1 + FCA_FILTER_MARGIN))) {
+                const R qcap = num / den;
+                if (qcap < qmag) {
+                    cap = 1;
+                    qmag = qcap;
+                }
+            }
+        }
+        qmag = (dL > (R)0) ? qmag : -qmag;
     }
-    return (dL > (R)0) ? qmag : -qmag;
+    return qmag;
 }
 
 /* Open-boundary virtual face of a LAND edge cell: z_b = z_a - S_bnd*dx,
  * h_b = 0, n_b = n_a, d = dx; inflow clipped to 0 (model.c:166-177). */
 template <class R>
-FCA_HD R open_flux(R za, R ha, R na, R dt, R den, const PhysK<R> &p, int &cap)
+FCA_HD R open_flux(R za, R ha, R na, R dt, R den, R rden, const PhysK<R> &p, int &cap)
 {
     R zb = za - p.S_bnd * p.dx;
-    R q = face_flux<R>(za, ha, na, FCA_LAND, zb, (R)0, na, FCA_LAND, p.dx, dt, den, p, cap);
+    R q = face_flux<R>(za, ha, na, FCA_LAND, zb, (R)0, na, FCA_LAND, p.dx, p.rdx, p.fdx, dt,
+                       den, rden, p, cap);
     return (q < (R)0) ? (R)0 : q;
 }
 
@@ -181,6 +328,31 @@ FCA_HD R level_of(int m, R z, R h, const PhysK<R> &p)
     return (m == FCA_SEA) ? p.z_sea : ((m == FCA_WALL) ? z : z + h);
 }
 
+/* Manning speed of a wet LAND cell from its largest slope, capped at the
+ * critical speed and floored at 1e-6 (physics.h:196-214). */
+template <class R>
+FCA_HD R cell_speed(R h, R n, R smax, const PhysK<R> &p)
+{
+    R v, vcr;
+    if (n < (R)1e-12) {
+        n = (R)1e-12;
+    }
+    if (smax < (R)0) {
+        smax = (R)0;
+    }
+    v = rcp_cr(n) * cbrt_sq(h) * sqrt_via_double(smax);
+    if (!(v < sqrt_rough(p.g * h) * ((R)1 - FCA_FILTER_MARGIN))) {
+        vcr = crit_speed(p.g, h);
+        if (v > vcr) {
+            v = vcr;
+        }
+    }
+    if (v < (R)1e-6) {
+        v = (R)1e-6;
+    }
+    return v;
+}
+
 /* Stable step of a wet LAND cell from its largest slope
  * (physics.h:193-217); the caller has checked h >= tau_h. */
 template <class R>
@@ -196,10 +368,12 @@ FCA_HD R cell_dt_from_smax(R h, R n, R smax, const PhysK<R> &p)
     if (smax < (R)0) {
         smax = (R)0;
     }
-    v = ((R)1 / n) * cbrt_sq(h) * sqrt_via_double(smax);
-    vcr = crit_speed(p.g, h);
-    if (v > vcr) {
-        v = vcr;
+    v = rcp_cr(n) * cbrt_sq(h) * sqrt_via_double(smax);
+    if (!(v < sqrt_rough(p.g * h) * ((R)1 - FCA_FILTER_MARGIN))) {
+        vcr = crit_speed(p.g, h);
+        if (v > vcr) {
+            v = vcr;
+        }
     }
     if (v < (R)1e-6) {
         v = (R)1e-6;
@@ -213,7 +387,7 @@ FCA_HD R cell_dt_from_smax(R h, R n, R smax, const PhysK<R> &p)
 template <class R>
 struct StepVals {
     double t, F, t_new;
-    R dt, dt_base, tmax_val, rain_f, den;
+    R dt, dt_base, tmax_val, rain_f, den, rden;
     int active, hit, need_cand, step;
 };
 
@@ -379,6 +553,7 @@ FCA_HD void ctrl_eval(const FcaDevCtrl &in, R cand, const FcaCtlParams &c, const
     sv.rain_f = (R)sv.F;
     sv.tmax_val = (R)(in.t + (double)dt);
     sv.den = (R)(2 * p.K) * dt;
+    sv.rden = (R)1 / sv.den;
     sv.t_new = hit ? t_next : in.t + (double)dt;
     sv.need_cand = ((in.step + 1) % (c.U > 0 ? c.U : 1)) == 0;
     nx.t = sv.t_new;
@@ -476,6 +651,8 @@ inline PhysK<R> make_phys(const FcaPhys &f)
     p.w = (f.K == 8) ? ((R)sqrt(2.0) - (R)1) * p.dx : p.dx;
     p.ddiag = (R)sqrt(2.0) * p.dx;
     p.U = f.U, p.K = f.K, p.open = f.boundary == FCA_BND_OPEN;
+    p.rdx = (R)1 / p.dx, p.rddiag = (R)1 / p.ddiag, p.rA = (R)1 / p.A;
+    p.fdx = p.fddiag = p.fA = 0;
     return p;
 }
 

@@ -187,8 +187,15 @@ struct FusedArgs {
     FcaAccSlot *slots;
 };
 
+/* Optional occupancy floor for the fused kernel (waves per SIMD). */
+#ifdef FCA_FUSED_MIN_WAVES
+#define FCA_FUSED_ATTR __attribute__((amdgpu_waves_per_eu(FCA_FUSED_MIN_WAVES, 8)))
+#else
+#define FCA_FUSED_ATTR
+#endif
+
 template <int K8>
-__global__ void __launch_bounds__(FCA_BLOCK) k_fused(FusedArgs a)
+__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR k_fused(FusedArgs a)
 {
     __shared__ StepVals<R> sh_sv;
     __shared__ int sh_hbuf;
@@ -211,11 +218,12 @@ __global__ void __launch_bounds__(FCA_BLOCK) k_fused(FusedArgs a)
 
     GpuLanes lp;
     lp.lane = lane;
-    if (tile < a.ntiles) {
+    walk_acc_init<R>(lp);
+    {
+        /* persistent wavefronts: wave w walks tiles w, w + W, w + 2W, ... in a
+         * fixed order, so every workgroup's slot sums the same cells each step */
         WalkArgs<R> wa;
-        const int strip = tile / a.ngroups, grp = tile - strip * a.ngroups;
-        const int r0 = strip * a.strip_rows;
-        const int r1 = (r0 + a.strip_rows < a.nr) ? r0 + a.strip_rows : a.nr;
+        const int W = (int)gridDim.x * (FCA_BLOCK / 64);
         wa.z = a.z, wa.n = a.n, wa.cr = a.cr, wa.mask = a.mask;
         wa.h_in = sh_hbuf ? a.h1 : a.h0;
         wa.h_out = sh_hbuf ? a.h0 : a.h1;
@@ -223,12 +231,14 @@ __global__ void __launch_bounds__(FCA_BLOCK) k_fused(FusedArgs a)
         wa.nr = a.nr, wa.nc = a.nc, wa.strip_rows = a.strip_rows;
         wa.p = a.p;
         wa.sv = sv;
-        walk_strip<R, K8>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
-    } else {
-        lp.s.v_rain = lp.s.clamp = lp.s.v_sea = lp.s.v_bnd = 0.0;
-        lp.s.cap = 0;
-        lp.s.dtc = (R)INFINITY;
+        for (int t = tile; t < a.ntiles; t += W) {
+            const int strip = t / a.ngroups, grp = t - strip * a.ngroups;
+            const int r0 = strip * a.strip_rows;
+            const int r1 = (r0 + a.strip_rows < a.nr) ? r0 + a.strip_rows : a.nr;
+            walk_strip<R, K8>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+        }
     }
+    walk_acc_finish<R>(lp, a.p);
 
     {
         double vr = wave_sum(lp.s.v_rain), cl = wave_sum(lp.s.clamp);
@@ -297,7 +307,8 @@ __global__ void __launch_bounds__(FCA_BLOCK) k_split_faces(SplitArgs a)
         int cap;
 #define FACE(A_, B_, D_)                                                                    \
     face_flux<R>(a.z[A_], a.h[A_], a.n[A_], a.mask[A_], a.z[B_], a.h[B_], a.n[B_], a.mask[B_], \
-                 (D_), sv.dt, sv.den, p, cap)
+                 (D_), (D_) == p.dx ? p.rdx : p.rddiag, (D_) == p.dx ? p.fdx : p.fddiag, sv.dt, \
+                 sv.den, sv.rden, p, cap)
         if (c + 1 < nc) {
             a.qx[(size_t)r * (nc - 1) + c] = FACE(i, i + 1, p.dx);
             cap_n += cap;
@@ -317,7 +328,7 @@ __global__ void __launch_bounds__(FCA_BLOCK) k_split_faces(SplitArgs a)
             const int on[4] = {c == 0, c == nc - 1, r == 0, r == nr - 1};
             for (int k = 0; k < 4; k++) {
                 if (on[k]) {
-                    a.qo[4 * i + k] = open_flux<R>(a.z[i], a.h[i], a.n[i], sv.dt, sv.den, p, cap);
+                    a.qo[4 * i + k] = open_flux<R>(a.z[i], a.h[i], a.n[i], sv.dt, sv.den, sv.rden, p, cap);
                     cap_n += cap;
                 }
             }
@@ -454,7 +465,7 @@ __global__ void __launch_bounds__(FCA_BLOCK) k_split_update(SplitArgs a)
         vb = (double)dt * (double)qos;
         if (land) {
             R rain = a.cr[i] * sv.rain_f;
-            R hn = a.h[i] + (dt * (net / A) + rain);
+            R hn = a.h[i] + (dt * div_c(net, A, a.p.rA, a.p.fA) + rain);
             vr = (double)A * (double)rain;
             if (hn < (R)0) {
                 cl = (double)(-hn) * (double)A;
@@ -607,6 +618,14 @@ __global__ void __launch_bounds__(FCA_BLOCK)
 
 /* ---- launchers (C ABI) ----------------------------------------------------- */
 
+/* PhysK of the run with the host-verified constant-divisor flags. */
+static PhysK<R> phys_of(const FcaDev *d)
+{
+    PhysK<R> p = make_phys<R>(d->phys);
+    p.fdx = d->fdiv[0], p.fddiag = d->fdiv[1], p.fA = d->fdiv[2];
+    return p;
+}
+
 static inline unsigned blocks_for(size_t n)
 {
     size_t b = (n + FCA_BLOCK - 1) / FCA_BLOCK;
@@ -640,6 +659,9 @@ extern "C" void fca_k_geometry(FcaDev *d)
     d->nstrips = (d->nr + d->strip_rows - 1) / d->strip_rows;
     d->ntiles = d->ngroups * d->nstrips;
     d->nblocks = (d->ntiles + FCA_BLOCK / 64 - 1) / (FCA_BLOCK / 64);
+    if (d->max_blocks > 0 && d->nblocks > d->max_blocks) {
+        d->nblocks = d->max_blocks;
+    }
 }
 
 extern "C" int fca_k_cand_reset(const FcaDev *d, int slot, hipStream_t st)
@@ -652,7 +674,7 @@ extern "C" int fca_k_cand_reset(const FcaDev *d, int slot, hipStream_t st)
 extern "C" int fca_k_cand_init(const FcaDev *d, int hbuf, int slot, hipStream_t st)
 {
     const size_t N = (size_t)d->nr * d->nc;
-    const PhysK<R> p = make_phys<R>(d->phys);
+    const PhysK<R> p = phys_of(d);
     Key *dst = (Key *)d->cand + slot;
     if (fca_k_cand_reset(d, slot, st) != 0) {
         return -1;
@@ -669,7 +691,7 @@ extern "C" int fca_k_cand_init(const FcaDev *d, int hbuf, int slot, hipStream_t 
 
 extern "C" int fca_k_step(const FcaDev *d, long long k, hipStream_t st)
 {
-    const PhysK<R> p = make_phys<R>(d->phys);
+    const PhysK<R> p = phys_of(d);
     if (d->kernel == FCA_HIP_FUSED) {
         FusedArgs a;
         a.z = d->z, a.n = d->n, a.cr = d->cr, a.mask = d->mask;

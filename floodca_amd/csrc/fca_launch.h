@@ -29,8 +29,11 @@ typedef struct FcaDev {
     int rec_cap;
     FcaPhys phys;
     FcaCtlParams ctl;
-    /* fused geometry */
-    int strip_rows, ngroups, nstrips, ntiles, nblocks;
+    /* a/d as two fmas after a multiply by RN(1/d) is exact for d = dx, sqrt2*dx,
+     * A: verified on the host for every float mantissa (div_const_exact) */
+    int fdiv[3];
+    /* fused geometry; max_blocks caps the persistent grid (0 = one wave per tile) */
+    int strip_rows, ngroups, nstrips, ntiles, nblocks, max_blocks;
     double *red;            /* scratch of the volume / ensemble reductions */
     int nred;
 } FcaDev;

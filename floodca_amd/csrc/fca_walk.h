@@ -66,10 +66,14 @@ struct WalkLane {
     int xm, xm2, xm3, xm4, xm5, xm6;
     R dz1, dz2, dz3, dz4, dh1, dh2, dh3, dh4;
     int dm1, dm2, dm3, dm4;
+    /* prefetched: mask of rows r+1 and r+2, z/h/n of row r+1, and cr/hmax of
+     * row r-1 (the next update row) */
+    int pm1, pm2;
+    R pz, ph, pn, pcr, phm;
     /* accumulators of this lane's output cells */
     double v_rain, clamp, v_sea, v_bnd;
     long long cap;
-    R dtc;
+    R dtc, vmax;
 };
 
 /* Walks one strip for one wavefront. strip covers output rows
@@ -81,7 +85,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
     const int nr = a.nr, nc = a.nc;
     const PhysK<R> &p = a.p;
     const StepVals<R> &sv = a.sv;
-    const R dt = sv.dt, den = sv.den, A = p.A;
+    const R dt = sv.dt, den = sv.den, rden = sv.rden, A = p.A;
     const int rb = (r_out0 - 3 > 0) ? r_out0 - 3 : 0;
     const int re = (r_out1 + 2 < nr + 2) ? r_out1 + 2 : nr + 2; /* inclusive */
     int r;
@@ -100,9 +104,19 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         s.oW0 = s.oE0 = s.oN0 = s.oS0 = s.oW1 = s.oE1 = s.oN1 = s.oS1 = (R)0;
         s.oW2 = s.oE2 = s.oN2 = s.oS2 = (R)0;
         s.s1 = s.s2 = s.s3 = (R)1;
-        s.v_rain = s.clamp = s.v_sea = s.v_bnd = 0.0;
-        s.cap = 0;
-        s.dtc = (R)INFINITY;
+        s.pcr = s.phm = (R)0;
+        s.pz = s.ph = s.pn = (R)0;
+        s.pm1 = s.pm2 = FCA_NONE;
+        if (s.colok && rb < nr) {
+            const size_t i = (size_t)rb * (size_t)nc + (size_t)s.c;
+            s.pm1 = a.mask[i];
+            if (s.pm1 == FCA_LAND) {
+                s.pz = a.z[i], s.ph = a.h_in[i], s.pn = a.n[i];
+            }
+        }
+        if (s.colok && rb + 1 < nr) {
+            s.pm2 = a.mask[(size_t)(rb + 1) * (size_t)nc + (size_t)s.c];
+        }
     });
 
     for (r = rb; r <= re; r++) {
@@ -111,19 +125,22 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         const int outB = r - 1 >= r_out0 && r - 1 < r_out1;
         const int qB = r - 1, qC = r - 2, qD = r - 3;
 
-        /* ---- A: read row r and form the faces touching it ---------------- */
+        /* ---- A: take row r (prefetched), prefetch ahead, form the faces ---- */
         lp.each([&](L &s, int) {
-            if (rowA && s.colok) {
-                size_t i = (size_t)r * (size_t)nc + (size_t)s.c;
-                s.m0 = a.mask[i];
-                if (s.m0 == FCA_LAND) {
-                    s.z0 = a.z[i];
-                    s.h0 = a.h_in[i];
-                    s.n0 = a.n[i];
-                }
-            } else {
-                s.m0 = FCA_NONE;
+            s.m0 = s.pm1;
+            if (s.m0 == FCA_LAND) {
+                s.z0 = s.pz, s.h0 = s.ph, s.n0 = s.pn;
             }
+            /* row r+1: z/h/n of LAND cells (mask fetched last iteration) */
+            s.pm1 = s.pm2;
+            if (s.colok && r + 1 < nr && s.pm1 == FCA_LAND) {
+                const size_t i = (size_t)(r + 1) * (size_t)nc + (size_t)s.c;
+                s.pz = a.z[i], s.ph = a.h_in[i], s.pn = a.n[i];
+            }
+            /* row r+2: mask */
+            s.pm2 = (s.colok && r + 2 < nr) ? (int)a.mask[(size_t)(r + 2) * (size_t)nc +
+                                                           (size_t)s.c]
+                                            : FCA_NONE;
         });
         lp.pull_right(&L::xz, &L::z0);
         lp.pull_right(&L::xh, &L::h0);
@@ -146,26 +163,26 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
                 return;
             }
             if (east) {
-                s.qx0 = face_flux<R>(s.z0, s.h0, s.n0, s.m0, s.xz, s.xh, s.xn, s.xm, p.dx, dt,
-                                     den, p, cap);
+                s.qx0 = face_flux<R>(s.z0, s.h0, s.n0, s.m0, s.xz, s.xh, s.xn, s.xm, p.dx, p.rdx,
+                                     p.fdx, dt, den, rden, p, cap);
                 if (outA && s.outcol) {
                     s.cap += cap;
                 }
             }
             if (r >= 1 && s.colok) {
-                s.qy1 = face_flux<R>(s.z1, s.h1, s.n1, s.m1, s.z0, s.h0, s.n0, s.m0, p.dx, dt,
-                                     den, p, cap);
+                s.qy1 = face_flux<R>(s.z1, s.h1, s.n1, s.m1, s.z0, s.h0, s.n0, s.m0, p.dx, p.rdx,
+                                     p.fdx, dt, den, rden, p, cap);
                 if (outB && s.outcol) {
                     s.cap += cap;
                 }
                 if (K8 && east) {
                     s.qd1_1 = face_flux<R>(s.z1, s.h1, s.n1, s.m1, s.xz, s.xh, s.xn, s.xm,
-                                           p.ddiag, dt, den, p, cap);
+                                           p.ddiag, p.rddiag, p.fddiag, dt, den, rden, p, cap);
                     if (outB && s.outcol) {
                         s.cap += cap;
                     }
                     s.qd2_1 = face_flux<R>(s.xq, s.xr, s.xs, s.xm2, s.z0, s.h0, s.n0, s.m0,
-                                           p.ddiag, dt, den, p, cap);
+                                           p.ddiag, p.rddiag, p.fddiag, dt, den, rden, p, cap);
                     if (outB && s.outcol) {
                         s.cap += cap;
                     }
@@ -175,16 +192,16 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
                 (s.c == 0 || s.c == nc - 1 || r == 0 || r == nr - 1)) {
                 int cw = 0, ce = 0, cn = 0, cs = 0;
                 if (s.c == 0) {
-                    s.oW0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, p, cw);
+                    s.oW0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, rden, p, cw);
                 }
                 if (s.c == nc - 1) {
-                    s.oE0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, p, ce);
+                    s.oE0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, rden, p, ce);
                 }
                 if (r == 0) {
-                    s.oN0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, p, cn);
+                    s.oN0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, rden, p, cn);
                 }
                 if (r == nr - 1) {
-                    s.oS0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, p, cs);
+                    s.oS0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, rden, p, cs);
                 }
                 if (outA && s.outcol) {
                     s.cap += cw + ce + cn + cs;
@@ -336,8 +353,8 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
                 {
                     size_t i = (size_t)qC * (size_t)nc + (size_t)s.c;
                     double clamp_here = 0.0;
-                    rain = a.cr[i] * sv.rain_f;
-                    hnew = s.h2 + (dt * (net / A) + rain);
+                    rain = s.pcr * sv.rain_f;
+                    hnew = s.h2 + (dt * div_c(net, A, p.rA, p.fA) + rain);
                     if (hnew < (R)0) {
                         clamp_here = (double)(-hnew) * (double)A;
                         hnew = (R)0;
@@ -347,7 +364,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
                         s.v_rain += (double)A * (double)rain;
                         s.clamp += clamp_here;
                         a.h_out[i] = hnew;
-                        if (hnew > a.hmax[i]) {
+                        if (hnew > s.phm) {
                             a.hmax[i] = hnew;
                             a.tmax[i] = sv.tmax_val;
                         }
@@ -355,6 +372,18 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
                 }
             });
         }
+
+        /* prefetch cr and hmax of row r-1, which the next iteration updates */
+        lp.each([&](L &s, int) {
+            const int q = r - 1;
+            if (s.colok && q >= 0 && q < nr && s.m1 == FCA_LAND) {
+                const size_t i = (size_t)q * (size_t)nc + (size_t)s.c;
+                s.pcr = a.cr[i];
+                if (s.outcol && q >= r_out0 && q < r_out1) {
+                    s.phm = a.hmax[i];
+                }
+            }
+        });
 
         /* ---- D: next step's stable time step on row r-3 ------------------ */
         if (sv.need_cand && qD >= r_out0 && qD < r_out1 && qD < nr) {
@@ -379,39 +408,55 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
                 lp.pull_left(&L::dm4, &L::m4);
             }
             lp.each([&](L &s, int) {
-                R la, smax = (R)0, sl, dta;
+                R la, ax = (R)0, smax, v;
                 if (!s.outcol || s.m3 != FCA_LAND || s.hn3 < p.tau_h) {
                     return;
                 }
                 la = s.z3 + s.hn3;
-#define FCA_SLOPE(cond, m_, z_, h_, d_)                                      \
-    if ((cond) && (m_) != FCA_WALL) {                                         \
-        sl = abs_slope<R>(la, level_of<R>((m_), (z_), (h_), p), (d_));        \
-        if (sl > smax) {                                                      \
-            smax = sl;                                                        \
-        }                                                                     \
+                /* largest |level difference| per face length: max_k RN(a_k/d)
+                 * = RN(max_k a_k / d) as rounding is monotonic, so one
+                 * division per length replaces one per neighbour */
+#define FCA_LEVEL_DIFF(cond, m_, z_, h_, acc_)                                \
+    if ((cond) && (m_) != FCA_WALL) {                                           \
+        R dl_ = la - level_of<R>((m_), (z_), (h_), p);                          \
+        if (dl_ < (R)0) {                                                       \
+            dl_ = -dl_;                                                         \
+        }                                                                       \
+        if (dl_ > acc_) {                                                       \
+            acc_ = dl_;                                                         \
+        }                                                                       \
     }
-                FCA_SLOPE(s.c + 1 < nc, s.xm, s.xz, s.xh, p.dx)
-                FCA_SLOPE(s.c > 0, s.xm2, s.xq, s.xr, p.dx)
-                FCA_SLOPE(qD + 1 < nr, s.m2, s.z2, s.hn2, p.dx)
-                FCA_SLOPE(qD > 0, s.m4, s.z4, s.hn4, p.dx)
-                if (K8) {
-                    FCA_SLOPE(qD + 1 < nr && s.c + 1 < nc, s.dm1, s.dz1, s.dh1, p.ddiag)
-                    FCA_SLOPE(qD + 1 < nr && s.c > 0, s.dm2, s.dz2, s.dh2, p.ddiag)
-                    FCA_SLOPE(qD > 0 && s.c + 1 < nc, s.dm3, s.dz3, s.dh3, p.ddiag)
-                    FCA_SLOPE(qD > 0 && s.c > 0, s.dm4, s.dz4, s.dh4, p.ddiag)
-                }
-#undef FCA_SLOPE
+                FCA_LEVEL_DIFF(s.c + 1 < nc, s.xm, s.xz, s.xh, ax)
+                FCA_LEVEL_DIFF(s.c > 0, s.xm2, s.xq, s.xr, ax)
+                FCA_LEVEL_DIFF(qD + 1 < nr, s.m2, s.z2, s.hn2, ax)
+                FCA_LEVEL_DIFF(qD > 0, s.m4, s.z4, s.hn4, ax)
                 if (p.open && (s.c == 0 || s.c == nc - 1 || qD == 0 || qD == nr - 1)) {
-                    R lv = s.z3 - p.S_bnd * p.dx;
-                    sl = abs_slope<R>(la, lv, p.dx);
-                    if (sl > smax) {
-                        smax = sl;
+                    R dl_ = la - (s.z3 - p.S_bnd * p.dx);
+                    if (dl_ < (R)0) {
+                        dl_ = -dl_;
+                    }
+                    if (dl_ > ax) {
+                        ax = dl_;
                     }
                 }
-                dta = cell_dt_from_smax<R>(s.hn3, s.n3, smax, p);
-                if (dta < s.dtc) {
-                    s.dtc = dta;
+                smax = div_c(ax, p.dx, p.rdx, p.fdx);
+                if (K8) {
+                    R ad = (R)0, sd;
+                    FCA_LEVEL_DIFF(qD + 1 < nr && s.c + 1 < nc, s.dm1, s.dz1, s.dh1, ad)
+                    FCA_LEVEL_DIFF(qD + 1 < nr && s.c > 0, s.dm2, s.dz2, s.dh2, ad)
+                    FCA_LEVEL_DIFF(qD > 0 && s.c + 1 < nc, s.dm3, s.dz3, s.dh3, ad)
+                    FCA_LEVEL_DIFF(qD > 0 && s.c > 0, s.dm4, s.dz4, s.dh4, ad)
+                    sd = div_c(ad, p.ddiag, p.rddiag, p.fddiag);
+                    if (sd > smax) {
+                        smax = sd;
+                    }
+                }
+#undef FCA_LEVEL_DIFF
+                /* dt_a = alpha*dx / v is non-increasing in v, so the lane keeps
+                 * the largest v and divides once at the end of the strip */
+                v = cell_speed<R>(s.hn3, s.n3, smax, p);
+                if (v > s.vmax) {
+                    s.vmax = v;
                 }
             });
         }
@@ -432,6 +477,28 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             s.s3 = s.s2, s.s2 = s.s1;
         });
     }
+}
+
+/* Zeroes a lane's accumulators; a wavefront may walk several strips. */
+template <class R, class LP>
+FCA_HD void walk_acc_init(LP &lp)
+{
+    lp.each([&](WalkLane<R> &s, int) {
+        s.v_rain = s.clamp = s.v_sea = s.v_bnd = 0.0;
+        s.cap = 0;
+        s.dtc = (R)INFINITY;
+        s.vmax = (R)0;
+    });
+}
+
+/* The lane's time-step candidate from its largest speed: dt_a = alpha*dx/v is
+ * non-increasing in v (alpha*dx > 0 is checked by the host). */
+template <class R, class LP>
+FCA_HD void walk_acc_finish(LP &lp, const PhysK<R> &p)
+{
+    lp.each([&](WalkLane<R> &s, int) {
+        s.dtc = (s.vmax > (R)0) ? (p.alpha * p.dx) / s.vmax : (R)INFINITY;
+    });
 }
 
 #endif

@@ -10,6 +10,7 @@
 #include "floodca_hip.h"
 #include "fca_launch.h"
 
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -122,6 +123,58 @@ static long count_faces(const FcaGrid *g, const FcaPhys *p)
 
 #define NRED 4096
 
+/* Whether q0 = a*r, q = fma(fma(-q0, d, a), r, q0) with r = RN(1/d) equals the
+ * correctly rounded a/d for every float a. Division, multiply and fma scale
+ * exactly with a power of two while no intermediate leaves the normal range,
+ * so one binade of mantissas decides it (the device applies the shortcut only
+ * for 2^-80 <= |a| <= 2^80). Float precision only. */
+static int div_const_exact(double dd)
+{
+#ifdef FCA_DOUBLE
+    (void)dd;
+    return 0;
+#else
+    const float d = (float)dd, r = 1.0f / d;
+    uint32_t m;
+    if (!(d > 0.0f) || d != d || d > 1e30f || d < 1e-30f) {
+        return 0;
+    }
+    for (m = 0; m < (1u << 23); m++) {
+        uint32_t bits = 0x3f800000u | m;
+        float a, q0, q;
+        memcpy(&a, &bits, 4);
+        q0 = a * r;
+        q = fmaf(fmaf(-q0, d, a), r, q0);
+        if (q != a / d) {
+            return 0;
+        }
+    }
+    return 1;
+#endif
+}
+
+/* The divisors of the step: dx, (real)sqrt(2.0)*dx (physics.h:45-48), A. */
+static void set_div_flags(FcaDev *d)
+{
+    const real dx = d->phys.dx, ddiag = (real)sqrt(2.0) * dx;
+    d->fdiv[0] = div_const_exact((double)dx);
+    d->fdiv[1] = d->phys.K == 8 ? div_const_exact((double)ddiag) : 0;
+    d->fdiv[2] = div_const_exact((double)d->phys.A);
+}
+
+/* Workgroups of the fused kernel that are resident at once: 4 waves per
+ * SIMD (its register budget), 4 SIMDs per CU, 4 waves per workgroup. */
+static int resident_blocks(void)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0) {
+        return 0;
+    }
+    return cus * 4;
+}
+
 static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
                      const FcaHipOpts *o)
 {
@@ -142,6 +195,8 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
     d->ctl.steps = cfg->steps;
     d->ctl.U = cfg->phys.U;
     d->ctl.n_force_days = cfg->n_force_days;
+    set_div_flags(d);
+    d->max_blocks = (o && o->max_blocks > 0) ? o->max_blocks : resident_blocks();
     fca_k_geometry(d);
     s->N = N;
     s->batch = (o && o->graph_batch > 0) ? o->graph_batch : 32;
@@ -238,6 +293,9 @@ static int check_cfg(const FcaGrid *g, const FcaRunCfg *cfg, const FcaHipOpts *o
     }
     if (cfg->phys.K != 4 && cfg->phys.K != 8) {
         return fail(-1, "K must be 4 or 8, not %d", cfg->phys.K);
+    }
+    if (!(cfg->phys.alpha * cfg->phys.dx > 0)) {
+        return fail(-1, "alpha*dx must be positive");
     }
     if (o && o->kernel != FCA_HIP_FUSED && o->kernel != FCA_HIP_SPLIT) {
         return fail(-1, "unknown kernel family %d", o->kernel);
@@ -713,6 +771,7 @@ int fca_hip_ens_run(FcaHipEns *e, FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc,
     /* per-scenario configuration: forcing shape, horizon, step count */
     memcpy(s->cfg, cfg, sizeof(FcaRunCfg));
     s->d.phys = cfg->phys;
+    set_div_flags(&s->d);
     s->d.ctl.T_sim = (double)cfg->T_sim;
     s->d.ctl.out_every = (double)cfg->out_every;
     s->d.ctl.D_h = (double)cfg->D_h;

@@ -88,10 +88,12 @@ def backend(prec: str = "f32") -> Backend:
     return _LIBS[prec]
 
 
-def opts(prec="f32", kernel=FCA_HIP_FUSED, strip_rows=0, graph_batch=0, record_steps=0):
+def opts(prec="f32", kernel=FCA_HIP_FUSED, strip_rows=0, graph_batch=0, record_steps=0,
+         max_blocks=0):
     o = abi(prec).FcaHipOpts()
     o.kernel, o.strip_rows, o.graph_batch, o.record_steps = (kernel, strip_rows, graph_batch,
                                                              record_steps)
+    o.max_blocks = max_blocks
     return o
 
 

@@ -246,7 +246,8 @@ typedef struct FcaHipOpts {
     int strip_rows;    /* fused: rows walked by one wavefront (default 256) */
     int graph_batch;   /* steps per captured hipGraph replay (default 32; 1 = no graph) */
     int record_steps;  /* > 0: keep a per-step log of the first record_steps steps */
-    int reserved[4];
+    int max_blocks;    /* fused: workgroups of the persistent grid (default: 4 per CU) */
+    int reserved[3];
 } FcaHipOpts;
 
 /* One executed step, as the controller saw it (fca_step, model.c:363-382). */

@@ -68,12 +68,23 @@ class Oracle(_Lib):
         L.or_run.restype = C.c_int
         L.or_run.argtypes = [C.POINTER(a.FcaGrid), C.POINTER(a.FcaRunCfg), C.POINTER(a.FcaAcc),
                              C.POINTER(a.FcaHipStepRec), C.c_int, C.POINTER(C.c_int)]
+        L.or_libm_range.restype = None
+        L.or_libm_range.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_float, C.c_void_p]
+        L.or_cbrt_sq_cr_mismatches.restype = C.c_long
+        L.or_cbrt_sq_cr_mismatches.argtypes = [C.c_uint32, C.c_uint32]
         L.or_ens_count.restype = None
         L.or_ens_count.argtypes = [C.POINTER(a.FcaGrid), C.c_int, C.POINTER(C.c_double),
                                    C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_double)]
 
     def flux(self, *args):
         return self.face_flux(self.lib.or_face_flux, *args)
+
+    def libm_range(self, kind: int, first: int, count: int, g: float = 9.81):
+        """Reference float-mode libm values for the floats with bit patterns
+        first..first+count-1 (kind 0 cbrt_sq, 1 sqrt, 2 critical speed)."""
+        out = np.empty(count, dtype=np.float32)
+        self.lib.or_libm_range(kind, first, count, g, out.ctypes.data)
+        return out
 
     def cbrt_sq(self, hf):
         return self.lib.or_cbrt_sq(hf)

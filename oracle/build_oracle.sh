@@ -19,7 +19,7 @@ PARITY=(-std=gnu11 -O2 -ffp-contract=off -fno-fast-math -fPIC -D_ISOC11_SOURCE)
 for prec in f32 f64; do
     def=()
     [[ $prec == f64 ]] && def=(-DFCA_DOUBLE)
-    "$CC" "${PARITY[@]}" "${def[@]}" -Wall -shared -I"$ROOT/include" \
+    "$CC" "${PARITY[@]}" "${def[@]}" -Wall -fopenmp -shared -I"$ROOT/include" \
         "$HERE/fca_oracle.c" -lm -o "$HERE/liboracle_${prec}.so"
 done
 

@@ -704,6 +704,76 @@ int or_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, FcaHipStepRec *log, in
     return 0;
 }
 
+/* ---- libm reference values over float bit ranges ----------------------- */
+
+/* out[i] = the reference's float-mode expression for the float with bit
+ * pattern first + i: kind 0 (float)cbrt((double)x*(double)x) (physics.h:19-24),
+ * kind 1 (float)sqrt((double)x) (physics.h:106), kind 2 the critical speed
+ * (float)sqrt((double)g*(double)x) (physics.h:108). glibc libm; these feed the
+ * exhaustive checks of the device implementations. */
+void or_libm_range(int kind, uint32_t first, uint32_t count, float g, float *out)
+{
+    long i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < (long)count; i++) {
+        uint32_t b = first + (uint32_t)i;
+        float x;
+        memcpy(&x, &b, 4);
+        if (kind == 0) {
+            out[i] = (float)cbrt((double)x * (double)x);
+        } else if (kind == 1) {
+            out[i] = (float)sqrt((double)x);
+        } else {
+            out[i] = (float)sqrt((double)g * (double)x);
+        }
+    }
+}
+
+/* Sign of m^3 - x, exact for m with <= 26 significant bits (m^2 exact, m^3
+ * as a double-double via fma) and x a double. */
+static int cube_sign(double m, double x)
+{
+    double m2 = m * m, p = m2 * m, e = fma(m2, m, -p), d = (p - x) + e;
+    return (d > 0) - (d < 0);
+}
+
+static float f32_bits_add(float f, int k)
+{
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    b += (uint32_t)k;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+/* Counts the floats in [first, first+count) for which the reference's
+ * (float)cbrt((double)x*x) differs from the correctly rounded x^(2/3):
+ * glibc's result is within one float ulp, and exact cube comparisons at the
+ * two neighbouring midpoints decide the correctly rounded value. */
+long or_cbrt_sq_cr_mismatches(uint32_t first, uint32_t count)
+{
+    long i, bad = 0;
+#pragma omp parallel for schedule(dynamic, 65536) reduction(+ : bad)
+    for (i = 0; i < (long)count; i++) {
+        uint32_t b = first + (uint32_t)i;
+        float x, g, cr, up, dn;
+        double sq;
+        memcpy(&x, &b, 4);
+        sq = (double)x * (double)x;
+        g = (float)cbrt(sq);
+        up = f32_bits_add(g, 1);
+        dn = f32_bits_add(g, -1);
+        cr = g;
+        if (cube_sign(((double)g + (double)up) * 0.5, sq) < 0) {
+            cr = up;
+        } else if (cube_sign(((double)dn + (double)g) * 0.5, sq) > 0) {
+            cr = dn;
+        }
+        bad += cr != g;
+    }
+    return bad;
+}
+
 /* ---- ensemble counts (src/hybrid/ens_common.c:166-188) ----------------- */
 
 void or_ens_count(const FcaGrid *g, int nthr, const double *thr, uint32_t *counts,

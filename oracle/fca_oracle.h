@@ -51,6 +51,13 @@ int or_step(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, FcaHipStepRec *rec);
 int or_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, FcaHipStepRec *log, int log_cap,
            int *n_logged);
 
+/* Reference float-mode libm expressions over a bit range (see fca_oracle.c). */
+void or_libm_range(int kind, uint32_t first, uint32_t count, float g, float *out);
+
+/* Floats in [first, first+count) whose float-mode cbrt_sq is not the
+ * correctly rounded x^(2/3) (see fca_oracle.c). */
+long or_cbrt_sq_cr_mismatches(uint32_t first, uint32_t count);
+
 /* Ensemble count of one scenario (src/hybrid/ens_common.c:166-188) with
  * uint32 counts: counts[j*N + a] += (LAND && hmax >= thr[j]). */
 void or_ens_count(const FcaGrid *g, int nthr, const double *thr, uint32_t *counts,

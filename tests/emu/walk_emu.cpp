@@ -177,11 +177,13 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
                 const int strip = t / ngroups, grp = t % ngroups;
                 const int r0 = strip * strip_rows;
                 const int r1 = (r0 + strip_rows < nr) ? r0 + strip_rows : nr;
+                walk_acc_init<R>(lp);
                 if (K8) {
                     walk_strip<R, 1>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
                 } else {
                     walk_strip<R, 0>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
                 }
+                walk_acc_finish<R>(lp, p);
                 for (int l = 0; l < FCA_WAVE; l++) {
                     acc->V_rain += lp.s[l].v_rain;
                     acc->clamp_err += lp.s[l].clamp;
