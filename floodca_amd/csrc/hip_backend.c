@@ -162,19 +162,6 @@ static void set_div_flags(FcaDev *d)
     d->fdiv[2] = div_const_exact((double)d->phys.A);
 }
 
-/* Workgroups of the fused kernel that are resident at once: 4 waves per
- * SIMD (its register budget), 4 SIMDs per CU, 4 waves per workgroup. */
-static int resident_blocks(void)
-{
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0) {
-        return 0;
-    }
-    return cus * 4;
-}
-
 static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
                      const FcaHipOpts *o)
 {
@@ -196,7 +183,7 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
     d->ctl.U = cfg->phys.U;
     d->ctl.n_force_days = cfg->n_force_days;
     set_div_flags(d);
-    d->max_blocks = (o && o->max_blocks > 0) ? o->max_blocks : resident_blocks();
+    d->max_blocks = (o && o->max_blocks > 0) ? o->max_blocks : 0;
     fca_k_geometry(d);
     s->N = N;
     s->batch = (o && o->graph_batch > 0) ? o->graph_batch : 32;

@@ -246,7 +246,8 @@ typedef struct FcaHipOpts {
     int strip_rows;    /* fused: rows walked by one wavefront (default 256) */
     int graph_batch;   /* steps per captured hipGraph replay (default 32; 1 = no graph) */
     int record_steps;  /* > 0: keep a per-step log of the first record_steps steps */
-    int max_blocks;    /* fused: workgroups of the persistent grid (default: 4 per CU) */
+    int max_blocks;    /* fused: cap on workgroups, each then walking several strips
+                        * (default 0: one strip per wavefront, hardware-dispatched) */
     int reserved[3];
 } FcaHipOpts;
 

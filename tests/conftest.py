@@ -44,3 +44,55 @@ def make_case(nr, nc, prec="f32", K=4, boundary=A.FCA_BND_OPEN, hyeto=A.FCA_HYET
 @pytest.fixture
 def case():
     return make_case
+
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+GOLDEN_CASES = ["g33x17_k4_open_uniform", "g33x17_k8_closed_uniform",
+                "g57x41_k4_open_triangle_walls", "g57x41_k8_open_steps_u3", "g7x5_k4_open_dry",
+                "g128x96_k4_open_uniform"]
+
+
+def load_golden(name, prec):
+    """A reference run from tests/golden: (abi, input grid, cfg, expected dict)."""
+    import json
+
+    from floodca_amd.grid import Grid
+
+    a = A.abi(prec)
+    f = np.load(os.path.join(GOLDEN, f"{name}_{prec}.npz"), allow_pickle=False)
+    g = Grid(z=f["z"], mask=f["mask"], n=f["n"], C=f["C"], Rday=f["Rday"], h=f["h0"].copy(),
+             hmax=f["hmax0"].copy(), tmax=f["tmax0"].copy(), dx=float(f["dx"]))
+    cfg = A.default_cfg(a)
+    for k, v in json.loads(str(f["cfg"])).items():
+        if k.startswith("phys."):
+            setattr(cfg.phys, k[5:], v)
+        else:
+            setattr(cfg, k, v)
+    exp = {"h": f["h"], "hmax": f["hmax"], "tmax": f["tmax"], "log": f["log"],
+           "acc": json.loads(str(f["acc"]))}
+    return a, g, cfg, exp
+
+
+def emu_lib(prec):
+    """ctypes handle of the host emulation of the fused walk (tests/emu)."""
+    import ctypes as C
+
+    a = A.abi(prec)
+    L = C.CDLL(os.path.join(ROOT, "tests", "emu", f"libwalk_emu_{prec}.so"))
+    L.emu_run.restype = C.c_int
+    L.emu_run.argtypes = [C.POINTER(a.FcaGrid), C.POINTER(a.FcaRunCfg), C.POINTER(a.FcaAcc),
+                          C.c_int, C.POINTER(a.FcaHipStepRec), C.c_int, C.POINTER(C.c_int)]
+    return L
+
+
+def emu_run(grid, cfg, acc, prec, strip_rows=0, cap=100000):
+    import ctypes as C
+
+    a = A.abi(prec)
+    L = emu_lib(prec)
+    g = grid.fca_grid(a)
+    rec = (a.FcaHipStepRec * cap)()
+    n = C.c_int()
+    assert L.emu_run(C.byref(g), C.byref(cfg), C.byref(acc), strip_rows, rec, cap,
+                     C.byref(n)) == 0
+    return acc, list(rec)[:min(n.value, cap)]
