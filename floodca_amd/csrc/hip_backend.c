@@ -662,16 +662,32 @@ int fca_hip_run(FcaGrid *g, FcaRunCfg *cfg, FcaAcc *acc, FcaTimers *tm)
 
 void fca_hip_device_name(char *name, size_t n)
 {
+    char buf[256];
+    char arch[64];
     hipDeviceProp_t prop;
     int dev = 0;
     if (!name || n == 0) {
         return;
     }
-    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
-        snprintf(name, n, "unknown");
-        return;
+    buf[0] = arch[0] = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetName(buf, (int)sizeof(buf), dev) != hipSuccess) {
+        buf[0] = 0;
     }
-    snprintf(name, n, "%s", prop.name);
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        snprintf(arch, sizeof(arch), "%s", prop.gcnArchName);
+        if (!buf[0]) {
+            snprintf(buf, sizeof(buf), "%s", prop.name);
+        }
+    }
+    if (!buf[0]) {
+        snprintf(buf, sizeof(buf), "AMD GPU");
+    }
+    if (arch[0]) {
+        snprintf(name, n, "%s (%s)", buf, arch);
+    } else {
+        snprintf(name, n, "%s", buf);
+    }
 }
 
 int fca_hip_select_device(int local_rank)
