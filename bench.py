@@ -149,6 +149,8 @@ def main() -> int:
     ap.add_argument("--kernel", choices=("fused", "split"), default="fused")
     ap.add_argument("--strip", type=int, default=0, help="rows per wavefront strip (0 = default)")
     ap.add_argument("--neigh", type=int, default=4, choices=(4, 8))
+    ap.add_argument("--no-land-form", action="store_true",
+                    help="fused walk in its general form only (same results)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -178,7 +180,8 @@ def main() -> int:
           f"device {backend(prec).device_name()}", file=sys.stderr, flush=True)
 
     timer = HipTimer()
-    with Sim(g, cfg, acc, prec=prec, kernel=kern, strip_rows=args.strip) as sim:
+    flags = A.FCA_HIP_NO_LAND_FORM if args.no_land_form else 0
+    with Sim(g, cfg, acc, prec=prec, kernel=kern, strip_rows=args.strip, flags=flags) as sim:
         stream = C.c_void_p(sim.stream)
         sim.advance(args.warmup)
         sim.sync()

@@ -20,6 +20,7 @@ FCA_BND_OPEN, FCA_BND_CLOSED = 0, 1
 FCA_HYETO_TRIANGLE, FCA_HYETO_UNIFORM = 0, 1
 FCA_MAX_FORCE_DAYS = 64
 FCA_HIP_FUSED, FCA_HIP_SPLIT = 0, 1
+FCA_HIP_NO_LAND_FORM = 1
 
 
 def _make(real):
@@ -80,7 +81,7 @@ def _make(real):
     class FcaHipOpts(C.Structure):
         _fields_ = [("kernel", C.c_int), ("strip_rows", C.c_int), ("graph_batch", C.c_int),
                     ("record_steps", C.c_int), ("max_blocks", C.c_int),
-                    ("reserved", C.c_int * 3)]
+                    ("flags", C.c_int), ("reserved", C.c_int * 2)]
 
     class FcaHipStepRec(C.Structure):
         _fields_ = [("t", C.c_double), ("F", C.c_double), ("dt_base", real), ("dt", real),

@@ -51,7 +51,49 @@ typedef struct FcaAccSlot {
     long long pad;
 } FcaAccSlot;
 
+/* Whether a run admits the LAND form of the fused walk. The thresholds keep
+ * the arguments of the face's divisions, square root and hf^(2/3) inside the
+ * ranges where the device shortcuts need no range test (|dL| >= tau_l >=
+ * 2^-80, RN(tau_l/d) >= 2^-96 for the longest face, hf >= tau_h >= 2^-15),
+ * and every LAND roughness in [1e-12, 2^100] lets the face roughness go
+ * without its clamp and its reciprocal without a range test. */
+static inline int fca_land_phys_ok(const FcaPhys *ph)
+{
+    const real dx = ph->dx, dmax = (ph->K == 8) ? (real)sqrt(2.0) * dx : dx;
+    return ph->tau_l >= (real)0x1p-80 && ph->tau_l / dmax >= (real)0x1p-96 &&
+           ph->tau_h >= (real)0x1p-15;
+}
+static inline int fca_land_n_ok(const real *n, const uint8_t *mask, size_t N)
+{
+    size_t i;
+    for (i = 0; i < N; i++) {
+        if (mask[i] == FCA_LAND && !(n[i] >= (real)1e-12 && n[i] <= (real)0x1p100)) {
+            return 0;
+        }
+    }
+    return 1;
+}
+static inline int fca_land_form_ok(const FcaPhys *ph, const real *n, const uint8_t *mask,
+                                   size_t N)
+{
+    return fca_land_phys_ok(ph) && fca_land_n_ok(n, mask, N);
+}
+
 #ifdef __cplusplus
+
+#if defined(__HIPCC__)
+#define FCA_HDI __host__ __device__ inline __attribute__((always_inline))
+#else
+#define FCA_HDI inline
+#endif
+
+/* Marks a block the compiler must branch around rather than evaluate
+ * unconditionally (its work is rare and costly). */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FCA_COLD_PATH() __asm__ volatile("")
+#else
+#define FCA_COLD_PATH() ((void)0)
+#endif
 
 /* Physical constants of one run in the kernel's precision, plus the derived
  * widths of physics.h:37-48. */
@@ -66,6 +108,9 @@ struct PhysK {
      * on the host to equal a/d for every float mantissa (div_const_exact). */
     R rdx, rddiag, rA;
     int fdx, fddiag, fA;
+    /* the LAND form may be used: fca_land_form_ok and, in float, fdx, fA
+     * and (K=8) fddiag */
+    int fast;
 };
 
 /* a/d for a run-constant d with reciprocal rd = RN(1/d). With ok set (the
@@ -76,9 +121,10 @@ FCA_HD float div_c(float a, float d, float rd, int ok)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     const float aa = __builtin_fabsf(a);
-    if (ok && aa >= 0x1p-80f && aa <= 0x1p80f) {
-        const float q0 = a * rd;
-        return __builtin_fmaf(__builtin_fmaf(-q0, d, a), rd, q0);
+    if (ok && ((aa >= 0x1p-80f && aa <= 0x1p80f) || aa == 0.0f)) {
+        const float q0 = a * rd; /* a signed zero stays itself */
+        const float q = __builtin_fmaf(__builtin_fmaf(-q0, d, a), rd, q0);
+        return (aa == 0.0f) ? q0 : q;
     }
 #else
     (void)rd, (void)ok;
@@ -88,6 +134,26 @@ FCA_HD float div_c(float a, float d, float rd, int ok)
 FCA_HD double div_c(double a, double d, double rd, int ok)
 {
     (void)rd, (void)ok;
+    return a / d;
+}
+
+/* div_c for a divisor whose flag is set and a dividend known to be at least
+ * 2^-80 in magnitude (the LAND form's |dL| >= tau_l). */
+FCA_HD float div_c_big(float a, float d, float rd)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__builtin_fabsf(a) <= 0x1p80f) {
+        const float q0 = a * rd;
+        return __builtin_fmaf(__builtin_fmaf(-q0, d, a), rd, q0);
+    }
+#else
+    (void)rd;
+#endif
+    return a / d;
+}
+FCA_HD double div_c_big(double a, double d, double rd)
+{
+    (void)rd;
     return a / d;
 }
 
@@ -106,6 +172,30 @@ FCA_HD float rcp_cr(float b)
     return 1.0f / b;
 }
 FCA_HD double rcp_cr(double b) { return 1.0 / b; }
+
+/* rcp_cr for b known to lie in [2^-100, 2^100]. */
+FCA_HD float rcp_in_range(float b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float y = __builtin_amdgcn_rcpf(b);
+    return __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+#else
+    return 1.0f / b;
+#endif
+}
+FCA_HD double rcp_in_range(double b) { return 1.0 / b; }
+
+FCA_HDI float fabs_r(float x) { return __builtin_fabsf(x); }
+FCA_HDI double fabs_r(double x) { return __builtin_fabs(x); }
+FCA_HDI float fmax_r(float a, float b) { return __builtin_fmaxf(a, b); }
+FCA_HDI double fmax_r(double a, double b) { return __builtin_fmax(a, b); }
+
+/* x if x > 0, else +0: the term an outflow sum adds for x. */
+template <class R>
+FCA_HDI R pos_part(R x)
+{
+    return (x > (R)0) ? x : (R)0;
+}
 
 /* Rough sqrt (relative error < 2^-21) used only to skip exact work. */
 FCA_HD float sqrt_rough(float x)
@@ -226,9 +316,96 @@ FCA_HD float sqrt_via_double(float x)
 }
 FCA_HD double sqrt_via_double(double x) { return sqrt(x); }
 
+/* Correctly rounded sqrt of x in [2^-96, 2^126]: the hardware square root is
+ * within one ulp, and the residuals of its two neighbours pick the correctly
+ * rounded one; no scaling or special-value handling is needed in this range
+ * (checked for every float of the range on the GPU, tests/test_gpu_math.py). */
+FCA_HD float sqrt_in_range(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = f32_step(s, -1), sp = f32_step(s, 1);
+    const float em = __builtin_fmaf(-sm, s, x);
+    const float ep = __builtin_fmaf(-sp, s, x);
+    const float t = (em <= 0.0f) ? sm : s;
+    return (ep > 0.0f) ? sp : t;
+#else
+    return (float)sqrt((double)x);
+#endif
+}
+FCA_HD double sqrt_in_range(double x) { return sqrt(x); }
+
 /* (real)sqrt((double)g*(double)h), the critical velocity (physics.h:108,207). */
 FCA_HD float crit_speed(float g, float h) { return (float)sqrt((double)g * (double)h); }
 FCA_HD double crit_speed(double g, double h) { return sqrt(g * h); }
+
+/* RN(num/den) for the step constant den > 0 with rden = RN(1/den). On the
+ * device: the estimate num*rden refined by one remainder step, then
+ * certified by its exact remainder r = num - q*den (exact because q is within
+ * an ulp of the quotient): q is the correctly rounded quotient when
+ * |r| < (1-2^-10) * den * u/2, u the ulp of q towards zero (no quotient of two
+ * floats lies exactly halfway). Anything else takes a plain division. */
+FCA_HDI float div_step_cr(float num, float den, float rden)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    float q = 0.0f;
+    int ok = 0;
+    if (__builtin_fabsf(num) >= 0x1p-80f && __builtin_fabsf(num) <= 0x1p80f) {
+        const float q0 = num * rden;
+        const float q1 = __builtin_fmaf(__builtin_fmaf(-q0, den, num), rden, q0);
+        const float r1 = __builtin_fmaf(-q1, den, num);
+        const float u = __builtin_fabsf(q1) - __builtin_fabsf(f32_step(q1, -1));
+        ok = __builtin_fabsf(r1) < u * (den * (0.5f - 0x1p-11f));
+        q = q1;
+    }
+    if (!ok) {
+        FCA_COLD_PATH();
+        q = num / den;
+    }
+    return q;
+#else
+    (void)rden;
+    return num / den;
+#endif
+}
+FCA_HDI double div_step_cr(double num, double den, double rden)
+{
+    (void)rden;
+    return num / den;
+}
+
+/* The level-equalisation cap Qcap = |dL|*A / ((2K)*dt) (physics.h:113-121). */
+template <class R>
+FCA_HDI R level_cap(R qmag, R adL, R dt, R den, R rden, const PhysK<R> &p, int &cap)
+{
+    if (dt > (R)0 && p.K > 0) {
+        const R qcap = div_step_cr(adL * p.A, den, rden);
+        if (qcap < qmag) {
+            cap = 1;
+            qmag = qcap;
+        }
+    }
+    return qmag;
+}
+
+/* min(v, vcr) with vcr = (real)sqrt((double)g*(double)hf) (physics.h:
+ * 108-111). The exact critical speed is only needed when v is not clearly
+ * below it: fl(v*v) < fl(fl(g*hf)*(1-2^-16)) implies v < sqrt(g*hf)*(1-2^-18)
+ * < vcr, as each rounding is within 2^-24 (2^-53 in double); g*hf >= 2^-100
+ * keeps the comparison clear of underflow. */
+template <class R>
+FCA_HDI R crit_cap(R v, R hf, const PhysK<R> &p)
+{
+    const R gh = p.g * hf;
+    if (!(v * v < gh * ((R)1 - (R)0x1p-16) && gh >= (R)0x1p-100)) {
+        FCA_COLD_PATH();
+        const R vcr = crit_speed(p.g, hf);
+        if (v > vcr) {
+            v = vcr;
+        }
+    }
+    return v;
+}
 
 /* Face flux a->b, positive = a to b (physics.h:51-125). den = (real)(2K)*dt
  * and rden = RN(1/den) are evaluated once per step by the caller; d is dx or
@@ -266,23 +443,34 @@ FCA_HD R face_flux(R za, R ha, R na, int ma, R zb, R hb, R nb, int mb, R d, R rd
             nf = (R)1e-12;
         }
         v = rcp_cr(nf) * cbrt_sq(hf) * sqrt_via_double(div_c(adL, d, rd, dok));
-        if (!(v < sqrt_rough(p.g * hf) * ((R)1 - FCA_FILTER_MARGIN))) {
-            const R vcr = crit_speed(p.g, hf);
-            if (v > vcr) {
-                v = vcr;
-            }
-        }
+        v = crit_cap<R>(v, hf, p);
         qmag = v * hf * p.w;
-        if (dt > (R)0 && p.K > 0) {
-            const R num = adL * p.A;
-            if (!(num * rden > qmag * ((R)1 + FCA_FILTER_MARGIN))) {
-                const R qcap = num / den;
-                if (qcap < qmag) {
-                    cap = 1;
-                    qmag = qcap;
-                }
-            }
-        }
+        qmag = level_cap<R>(qmag, adL, dt, den, rden, p, cap);
+        qmag = (dL > (R)0) ? qmag : -qmag;
+    }
+    return qmag;
+}
+
+/* Face flux between two LAND cells given their beds z and levels l = z + h,
+ * for runs where fca_land_form_ok holds: the expressions of face_flux with
+ * the wall, sea and clamp terms (all inactive) left out. */
+template <class R>
+FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, R rden,
+                    const PhysK<R> &p, int &cap)
+{
+    const R dL = la - lb;
+    /* |dL| as a magnitude: where face_flux keeps a -0 this gives +0, and
+     * either zero is below tau_l, so the face is dead in both */
+    const R adL = fabs_r(dL);
+    const R hf = ((la > lb) ? la : lb) - ((za > zb) ? za : zb);
+    R qmag = (R)0;
+    cap = 0;
+    if (!((adL < p.tau_l) | (hf < p.tau_h))) {
+        const R nf = (R)0.5 * (na + nb);
+        R v = rcp_in_range(nf) * cbrt_sq(hf) * sqrt_in_range(div_c_big(adL, d, rd));
+        v = crit_cap<R>(v, hf, p);
+        qmag = v * hf * p.w;
+        qmag = level_cap<R>(qmag, adL, dt, den, rden, p, cap);
         qmag = (dL > (R)0) ? qmag : -qmag;
     }
     return qmag;
@@ -305,6 +493,7 @@ FCA_HD R limiter_s(R out_vol, R h, R A)
 {
     R hold = h * A;
     if (out_vol > hold && out_vol > (R)0) {
+        FCA_COLD_PATH();
         return hold / out_vol;
     }
     return (R)1;
@@ -333,7 +522,7 @@ FCA_HD R level_of(int m, R z, R h, const PhysK<R> &p)
 template <class R>
 FCA_HD R cell_speed(R h, R n, R smax, const PhysK<R> &p)
 {
-    R v, vcr;
+    R v;
     if (n < (R)1e-12) {
         n = (R)1e-12;
     }
@@ -341,12 +530,7 @@ FCA_HD R cell_speed(R h, R n, R smax, const PhysK<R> &p)
         smax = (R)0;
     }
     v = rcp_cr(n) * cbrt_sq(h) * sqrt_via_double(smax);
-    if (!(v < sqrt_rough(p.g * h) * ((R)1 - FCA_FILTER_MARGIN))) {
-        vcr = crit_speed(p.g, h);
-        if (v > vcr) {
-            v = vcr;
-        }
-    }
+    v = crit_cap<R>(v, h, p);
     if (v < (R)1e-6) {
         v = (R)1e-6;
     }
@@ -358,27 +542,10 @@ FCA_HD R cell_speed(R h, R n, R smax, const PhysK<R> &p)
 template <class R>
 FCA_HD R cell_dt_from_smax(R h, R n, R smax, const PhysK<R> &p)
 {
-    R v, vcr;
     if (h < p.tau_h) {
         return p.dt_max;
     }
-    if (n < (R)1e-12) {
-        n = (R)1e-12;
-    }
-    if (smax < (R)0) {
-        smax = (R)0;
-    }
-    v = rcp_cr(n) * cbrt_sq(h) * sqrt_via_double(smax);
-    if (!(v < sqrt_rough(p.g * h) * ((R)1 - FCA_FILTER_MARGIN))) {
-        vcr = crit_speed(p.g, h);
-        if (v > vcr) {
-            v = vcr;
-        }
-    }
-    if (v < (R)1e-6) {
-        v = (R)1e-6;
-    }
-    return p.alpha * p.dx / v;
+    return p.alpha * p.dx / cell_speed<R>(h, n, smax, p);
 }
 
 /* ---- the step controller (model.c:363-382, 599-601; rain.c:13-106) ------ */
@@ -653,6 +820,7 @@ inline PhysK<R> make_phys(const FcaPhys &f)
     p.U = f.U, p.K = f.K, p.open = f.boundary == FCA_BND_OPEN;
     p.rdx = (R)1 / p.dx, p.rddiag = (R)1 / p.ddiag, p.rA = (R)1 / p.A;
     p.fdx = p.fddiag = p.fA = 0;
+    p.fast = 0;
     return p;
 }
 

@@ -28,15 +28,16 @@ typedef CandKey<R>::type Key;
 
 /* ---- wavefront shifts ---------------------------------------------------- */
 
-/* DPP wave_shl:1 / wave_shr:1: lane i reads lane i+1 / i-1; lanes with no
- * source read 0. */
+/* DPP wave_shl:1 / wave_shr:1: lane i reads lane i+1 / i-1; with bound_ctrl
+ * lanes with no source read 0, and no prior value of the destination is
+ * needed. */
 __device__ __forceinline__ int lane_from_right(int x)
 {
-    return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false);
+    return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int lane_from_left(int x)
 {
-    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
+    return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ float lane_from_right(float x)
 {
@@ -79,6 +80,41 @@ struct GpuLanes {
     __device__ __forceinline__ void pull_left(T WalkLane<R>::*dst, T WalkLane<R>::*src)
     {
         s.*dst = lane_from_left(s.*src);
+    }
+    __device__ __forceinline__ void count_form(int) {}
+    /* Orders the wait for the loads of the row just taken (and of the masks
+     * and cr/hmax fetched with them) before any later memory access: the
+     * asm reads their registers, so the compiler's wait precedes it, and its
+     * memory clobber keeps the following stores behind it. */
+    __device__ __forceinline__ void loads_settled()
+    {
+        __asm__ volatile("" ::"v"(s.lv0), "v"(s.n0), "v"(s.pm1), "v"(s.pcr), "v"(s.phm)
+                         : "memory");
+    }
+    /* uniform: every lane's mask field is LAND */
+    __device__ __forceinline__ bool all_land(int WalkLane<R>::*m)
+    {
+        return __builtin_amdgcn_ballot_w64(s.*m != FCA_LAND) == 0;
+    }
+    /* uniform: no lane's mask field is LAND */
+    __device__ __forceinline__ bool no_land(int WalkLane<R>::*m)
+    {
+        return __builtin_amdgcn_ballot_w64(s.*m == FCA_LAND) == 0;
+    }
+    /* uniform: some lane's flag is set */
+    __device__ __forceinline__ bool any(int WalkLane<R>::*f)
+    {
+        return __builtin_amdgcn_ballot_w64(s.*f != 0) != 0;
+    }
+    /* every lane's dst = the largest src over the wavefront */
+    __device__ __forceinline__ void wave_max_to(R WalkLane<R>::*dst, R WalkLane<R>::*src)
+    {
+        R v = s.*src;
+        for (int off = 32; off > 0; off >>= 1) {
+            const R o = __shfl_xor(v, off, 64);
+            v = (o > v) ? o : v;
+        }
+        s.*dst = v;
     }
 };
 
@@ -202,7 +238,10 @@ __global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR k_fused(FusedArgs a)
     __shared__ double sh_acc[FCA_BLOCK / 64][4];
     __shared__ long long sh_cap[FCA_BLOCK / 64];
     __shared__ R sh_dtc[FCA_BLOCK / 64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    /* the wavefront index is uniform across the wavefront: readfirstlane
+     * lets the walk's row loop, row addresses and bounds live in scalar
+     * registers */
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int tile = blockIdx.x * (FCA_BLOCK / 64) + wave;
 
     if (threadIdx.x == 0) {
@@ -623,6 +662,7 @@ static PhysK<R> phys_of(const FcaDev *d)
 {
     PhysK<R> p = make_phys<R>(d->phys);
     p.fdx = d->fdiv[0], p.fddiag = d->fdiv[1], p.fA = d->fdiv[2];
+    p.fast = d->land_form;
     return p;
 }
 

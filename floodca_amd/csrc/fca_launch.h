@@ -32,6 +32,9 @@ typedef struct FcaDev {
     /* a/d as two fmas after a multiply by RN(1/d) is exact for d = dx, sqrt2*dx,
      * A: verified on the host for every float mantissa (div_const_exact) */
     int fdiv[3];
+    /* the fused walk may take its LAND form (fca_land_form_ok and the
+     * division flags it relies on) */
+    int land_form;
     /* fused geometry; max_blocks caps the persistent grid (0 = one wave per tile) */
     int strip_rows, ngroups, nstrips, ntiles, nblocks, max_blocks;
     double *red;            /* scratch of the volume / ensemble reductions */

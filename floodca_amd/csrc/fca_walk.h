@@ -5,24 +5,37 @@
  * rows. Lane l holds column c = c_first + l; the three lanes at each side are
  * halo, so a wavefront writes 58 columns. At row r the walk
  *   A. reads row r (z, h, n, mask) and forms the faces that touch it: the
- *      east face of row r, the face between rows r-1 and r, the two K=8
- *      diagonals between them, and the open-boundary faces of row r;
- *   B. sums the outflow of row r-1 and its limiter s (model.c:180-238,488-502);
+ *      east face of row r, the face between rows r-1 and r and the two K=8
+ *      diagonals between them (physics.h:51-125, model.c:393-482);
+ *   B. sums the outflow of row r-1, with its open-boundary faces, and its
+ *      limiter s (model.c:158-238, 488-502);
  *   C. updates row r-2: upwind-scaled net inflow, rain, clamp, hmax/tmax and
  *      the FP64 balance terms (model.c:241-319, 511-597);
- *   D. evaluates the next step's stable time step on row r-3 from the new
+ *   D. bounds the next step's stable time step on row r-3 from the new
  *      depths (model.c:111-156, physics.h:158-217).
  * Rows r-1..r-4 stay in a register ring, so every field crosses HBM once per
  * step; horizontal neighbours come from the adjacent lanes.
+ *
+ * Two forms of a row. Where rows r-4..r are LAND on all 64 lanes, away from
+ * the grid's edges, a row takes the LAND form: no wall, sea or open-boundary
+ * terms, levels formed once per cell. Any other row takes the general form,
+ * which keeps every rule. Both evaluate the same expressions in the same
+ * order, so the choice never changes a result.
+ *
+ * Stage D keeps, per wavefront, the largest exact cell speed found so far
+ * (vthr). A cell whose speed is bounded above by vthr cannot lower the
+ * step's minimum time step (alpha*dx / v is non-increasing in v), so its
+ * exact speed is only evaluated when a cheap upper bound exceeds vthr.
  *
  * The walk is a template over a lane policy LP: on the GPU a lane is one
  * work-item and pull_left/pull_right are DPP wavefront shifts; the CPU
  * emulation runs 64 lanes in lockstep with array shifts, so the CPU test
  * suite checks the same dataflow bit for bit against the oracle.
  *
- * Lane validity: a value pulled from outside the wavefront is 0. Faces are
- * valid on lanes 0..62, limiters on 1..62, new depths on 2..61 and time-step
- * candidates on 3..60, which are the lanes a wavefront writes.
+ * Lane validity: faces are valid on lanes 0..62, limiters on 1..62, new
+ * depths on 2..61 and time-step candidates on 3..60, which are the lanes a
+ * wavefront writes. Values a lane pulls from outside the wavefront only
+ * reach lanes that are not valid.
  */
 #ifndef FCA_WALK_H
 #define FCA_WALK_H
@@ -50,98 +63,172 @@ struct WalkArgs {
 template <class R>
 struct WalkLane {
     int c, colok, outcol;
+    uint32_t boff; /* byte offset of column c in a row of R */
     R z0, z1, z2, z3, z4;
     int m0, m1, m2, m3, m4;
     R h0, h1, h2;
     R n0, n1, n2, n3;
+    R lv0, lv1; /* z+h of rows r, r-1 (meaningful on LAND cells) */
     R hn2, hn3, hn4;
     R qx0, qx1, qx2;
     R qy1, qy2, qy3;       /* faces (r-1 -> r), (r-2 -> r-1), (r-3 -> r-2) */
     R qd1_1, qd1_2, qd1_3; /* SE diagonal faces (q,c)->(q+1,c+1), q = r-1.. */
     R qd2_1, qd2_2, qd2_3; /* SW diagonal faces (q,c+1)->(q+1,c), q = r-1.. */
-    R oW0, oE0, oN0, oS0, oW1, oE1, oN1, oS1, oW2, oE2, oN2, oS2;
     R s1, s2, s3;
     /* values pulled from neighbour lanes */
-    R xz, xh, xn, xq, xr, xs, xt, xu, xv, xw, xy, xa;
-    int xm, xm2, xm3, xm4, xm5, xm6;
+    R xz, xh, xn, xl, xq, xr, xs, xl1, xt, xu, xv, xw, xy, xa;
+    int xm, xm2, xm3, xm4;
     R dz1, dz2, dz3, dz4, dh1, dh2, dh3, dh4;
     int dm1, dm2, dm3, dm4;
-    /* prefetched: mask of rows r+1 and r+2, z/h/n of row r+1, and cr/hmax of
-     * row r-1 (the next update row) */
+    /* prefetched: mask of rows r+1 and r+2, z/h/n of row r+1, cr/hmax of row
+     * r-2 (this row's update) and of row r-1 (the next one) */
     int pm1, pm2;
-    R pz, ph, pn, pcr, phm;
+    R pz, ph, pn, pcr, phm, pcr1, phm1;
+    /* stores of row r-3 deferred to the next row's memory block: bit 0 h,
+     * bit 1 hmax and tmax */
+    int stq;
     /* accumulators of this lane's output cells */
     double v_rain, clamp, v_sea, v_bnd;
     long long cap;
-    R dtc, vmax;
+    R dtc, vmax, vthr;
+    /* stage D: largest |level difference| per face length, and whether the
+     * exact speed is needed */
+    R dax, dad;
+    int dneed;
 };
 
-/* Walks one strip for one wavefront. strip covers output rows
- * [r_out0, r_out1) and columns [c_first + FCA_HALO, c_first + FCA_HALO + 58). */
-template <class R, int K8, class LP>
-FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int c_first)
+/* Loads the element at byte offset boff of a row: a uniform row address plus
+ * a 32-bit lane offset. */
+template <class T>
+FCA_HDI T ld_row(const T *row, uint32_t boff)
+{
+    return *(const T *)((const char *)row + boff);
+}
+template <class T>
+FCA_HDI void st_row(T *row, uint32_t boff, T v)
+{
+    *(T *)((char *)row + boff) = v;
+}
+
+/* What a row of the walk needs besides the lane state. */
+struct WalkRow {
+    int r, r_out0, r_out1, c_first;
+    int edge_cols; /* the wavefront holds column 0 or nc-1 */
+};
+
+/* Upper bound test for stage D: whether a wet LAND cell's exact speed
+ * (cell_speed) might exceed thr > 0, from its depth h, roughness n and an
+ * upper bound s_hi of its largest slope (flat: the slope is exactly 0).
+ * v = RN(RN(RN(1/n')*h^(2/3))*sqrt(s)), n' = max(n, 1e-12), is within
+ * (1+2^-24)^5 of (1/n') h^(2/3) s^(1/2); comparing sixth powers,
+ * h^4 s^3 < (thr*n')^6, avoids every root. With h >= 2^-20 and s_hi >= 2^-30
+ * every partial product is a normal number, so each side is within 2^-20 of
+ * its exact value and the margin 2^-12 covers both. The critical speed bounds
+ * v too, so a cell whose vcr is below thr is skipped as well. A flat cell's speed is
+ * the floor 1e-6, which never exceeds a thr >= 1e-6 (a thr found by the walk
+ * is an exact speed, hence at least 1e-6). */
+template <class R>
+FCA_HDI int speed_may_exceed(R h, R n, R s_hi, int flat, R thr, R g)
+{
+    if (!(thr > (R)0)) {
+        return 1;
+    }
+    if (flat) {
+        return !(thr >= (R)1e-6);
+    }
+    /* the critical speed caps v: vcr = RN(sqrt(g*h)) <= thr when
+     * fl(g*h)*(1+2^-20) <= fl(thr*thr) (each side within 2^-23) */
+    if (g * h * ((R)1 + (R)0x1p-20) <= thr * thr && thr >= (R)1e-6 && thr <= (R)0x1p60) {
+        return 0;
+    }
+    {
+        const R nn = (n < (R)1e-12) ? (R)1e-12 : n;
+        const R h2 = h * h, s2 = s_hi * s_hi;
+        const R lhs = ((h2 * h2) * (s2 * s_hi)) * ((R)1 + (R)0x1p-12);
+        const R tn = thr * nn, tn2 = tn * tn;
+        const R rhs = (tn2 * tn2) * tn2;
+        return !(h >= (R)0x1p-20 && s_hi >= (R)0x1p-30 && lhs >= (R)0x1p-100 && lhs < rhs);
+    }
+}
+
+/* Open-boundary faces of a LAND cell of row q (model.c:166-177). */
+template <class R>
+struct OpenFaces {
+    R w, e, n, s;
+    int cap;
+};
+template <class R>
+FCA_HDI OpenFaces<R> open_faces(R z, R h, R n, int m, int c, int q, int nr, int nc, R dt, R den,
+                               R rden, const PhysK<R> &p)
+{
+    OpenFaces<R> o;
+    int cw = 0, ce = 0, cn = 0, cs = 0;
+    o.w = o.e = o.n = o.s = (R)0;
+    if (m == FCA_LAND) {
+        if (c == 0) {
+            o.w = open_flux<R>(z, h, n, dt, den, rden, p, cw);
+        }
+        if (c == nc - 1) {
+            o.e = open_flux<R>(z, h, n, dt, den, rden, p, ce);
+        }
+        if (q == 0) {
+            o.n = open_flux<R>(z, h, n, dt, den, rden, p, cn);
+        }
+        if (q == nr - 1) {
+            o.s = open_flux<R>(z, h, n, dt, den, rden, p, cs);
+        }
+    }
+    o.cap = cw + ce + cn + cs;
+    return o;
+}
+
+/* One row of the walk after row r's inputs were taken into the ring.
+ * LANDF: rows r-4..r are LAND on every lane, 1 <= r-4, r <= nr-2 and no lane
+ * holds column 0 or nc-1. */
+template <class R, int K8, int LANDF, class LP>
+FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
 {
     typedef WalkLane<R> L;
-    const int nr = a.nr, nc = a.nc;
+    const int nr = a.nr, nc = a.nc, r = w.r;
     const PhysK<R> &p = a.p;
     const StepVals<R> &sv = a.sv;
     const R dt = sv.dt, den = sv.den, rden = sv.rden, A = p.A;
-    const int rb = (r_out0 - 3 > 0) ? r_out0 - 3 : 0;
-    const int re = (r_out1 + 2 < nr + 2) ? r_out1 + 2 : nr + 2; /* inclusive */
-    int r;
+    const int rowA = r < nr;                            /* row r exists */
+    const int outA = r >= w.r_out0 && r < w.r_out1;    /* row r is written here */
+    const int qB = r - 1, qC = r - 2, qD = r - 3;
+    const int outB = qB >= w.r_out0 && qB < w.r_out1;
 
-    lp.each([&](L &s, int lane) {
-        s.c = c_first + lane;
-        s.colok = s.c >= 0 && s.c < nc;
-        s.outcol = s.colok && lane >= FCA_HALO && lane < FCA_WAVE - FCA_HALO;
-        s.z0 = s.z1 = s.z2 = s.z3 = s.z4 = (R)0;
-        s.m0 = s.m1 = s.m2 = s.m3 = s.m4 = FCA_NONE;
-        s.h0 = s.h1 = s.h2 = (R)0;
-        s.n0 = s.n1 = s.n2 = s.n3 = (R)0;
-        s.hn2 = s.hn3 = s.hn4 = (R)0;
-        s.qx0 = s.qx1 = s.qx2 = s.qy1 = s.qy2 = s.qy3 = (R)0;
-        s.qd1_1 = s.qd1_2 = s.qd1_3 = s.qd2_1 = s.qd2_2 = s.qd2_3 = (R)0;
-        s.oW0 = s.oE0 = s.oN0 = s.oS0 = s.oW1 = s.oE1 = s.oN1 = s.oS1 = (R)0;
-        s.oW2 = s.oE2 = s.oN2 = s.oS2 = (R)0;
-        s.s1 = s.s2 = s.s3 = (R)1;
-        s.pcr = s.phm = (R)0;
-        s.pz = s.ph = s.pn = (R)0;
-        s.pm1 = s.pm2 = FCA_NONE;
-        if (s.colok && rb < nr) {
-            const size_t i = (size_t)rb * (size_t)nc + (size_t)s.c;
-            s.pm1 = a.mask[i];
-            if (s.pm1 == FCA_LAND) {
-                s.pz = a.z[i], s.ph = a.h_in[i], s.pn = a.n[i];
-            }
+    /* ---- A: the faces that touch row r ---------------------------------- */
+    if (LANDF) {
+        lp.pull_right(&L::xz, &L::z0);
+        lp.pull_right(&L::xl, &L::lv0);
+        lp.pull_right(&L::xn, &L::n0);
+        if (K8) {
+            lp.pull_right(&L::xq, &L::z1);
+            lp.pull_right(&L::xl1, &L::lv1);
+            lp.pull_right(&L::xs, &L::n1);
         }
-        if (s.colok && rb + 1 < nr) {
-            s.pm2 = a.mask[(size_t)(rb + 1) * (size_t)nc + (size_t)s.c];
-        }
-    });
-
-    for (r = rb; r <= re; r++) {
-        const int rowA = r < nr;                       /* row r exists */
-        const int outA = r >= r_out0 && r < r_out1;    /* row r is written here */
-        const int outB = r - 1 >= r_out0 && r - 1 < r_out1;
-        const int qB = r - 1, qC = r - 2, qD = r - 3;
-
-        /* ---- A: take row r (prefetched), prefetch ahead, form the faces ---- */
         lp.each([&](L &s, int) {
-            s.m0 = s.pm1;
-            if (s.m0 == FCA_LAND) {
-                s.z0 = s.pz, s.h0 = s.ph, s.n0 = s.pn;
+            int cap, ct = 0;
+            s.qx0 = face_land<R>(s.z0, s.lv0, s.n0, s.xz, s.xl, s.xn, p.dx, p.rdx, dt, den, rden,
+                                 p, cap);
+            ct += outA ? cap : 0;
+            s.qy1 = face_land<R>(s.z1, s.lv1, s.n1, s.z0, s.lv0, s.n0, p.dx, p.rdx, dt, den,
+                                 rden, p, cap);
+            ct += outB ? cap : 0;
+            if (K8) {
+                s.qd1_1 = face_land<R>(s.z1, s.lv1, s.n1, s.xz, s.xl, s.xn, p.ddiag, p.rddiag,
+                                       dt, den, rden, p, cap);
+                ct += outB ? cap : 0;
+                s.qd2_1 = face_land<R>(s.xq, s.xl1, s.xs, s.z0, s.lv0, s.n0, p.ddiag, p.rddiag,
+                                       dt, den, rden, p, cap);
+                ct += outB ? cap : 0;
             }
-            /* row r+1: z/h/n of LAND cells (mask fetched last iteration) */
-            s.pm1 = s.pm2;
-            if (s.colok && r + 1 < nr && s.pm1 == FCA_LAND) {
-                const size_t i = (size_t)(r + 1) * (size_t)nc + (size_t)s.c;
-                s.pz = a.z[i], s.ph = a.h_in[i], s.pn = a.n[i];
+            if (s.outcol) {
+                s.cap += ct;
             }
-            /* row r+2: mask */
-            s.pm2 = (s.colok && r + 2 < nr) ? (int)a.mask[(size_t)(r + 2) * (size_t)nc +
-                                                           (size_t)s.c]
-                                            : FCA_NONE;
         });
+    } else {
         lp.pull_right(&L::xz, &L::z0);
         lp.pull_right(&L::xh, &L::h0);
         lp.pull_right(&L::xn, &L::n0);
@@ -158,7 +245,6 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             s.qx0 = (R)0;
             s.qy1 = (R)0;
             s.qd1_1 = s.qd2_1 = (R)0;
-            s.oW0 = s.oE0 = s.oN0 = s.oS0 = (R)0;
             if (!rowA) {
                 return;
             }
@@ -188,236 +274,247 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
                     }
                 }
             }
-            if (p.open && s.colok && s.m0 == FCA_LAND &&
-                (s.c == 0 || s.c == nc - 1 || r == 0 || r == nr - 1)) {
-                int cw = 0, ce = 0, cn = 0, cs = 0;
-                if (s.c == 0) {
-                    s.oW0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, rden, p, cw);
+        });
+    }
+
+    /* ---- B: outflow volume and limiter of row r-1 ---------------------- */
+    lp.pull_left(&L::xt, &L::qx1);
+    if (K8) {
+        lp.pull_left(&L::xu, &L::qd1_2);
+        lp.pull_left(&L::xv, &L::qd2_1);
+    }
+    if (LANDF) {
+        /* out += q for q > 0, in the order E, W, S, N[, SE, NW, SW, NE]; adding
+         * a zero instead of skipping leaves the (non-negative) sum unchanged */
+        lp.each([&](L &s, int) {
+            R out = pos_part(s.qx1) + pos_part(-s.xt);
+            out = out + pos_part(s.qy1);
+            out = out + pos_part(-s.qy2);
+            if (K8) {
+                out = out + pos_part(s.qd1_1);
+                out = out + pos_part(-s.xu);
+                out = out + pos_part(s.xv);
+                out = out + pos_part(-s.qd2_2);
+            }
+            out = out * dt;
+            s.s1 = limiter_s<R>(out, s.h1, A);
+        });
+    } else if (qB >= 0 && qB < nr) {
+        const int open_row = p.open && (w.edge_cols || qB == 0 || qB == nr - 1);
+        lp.each([&](L &s, int) {
+            R out = (R)0;
+            if (!s.colok) {
+                s.s1 = (R)1;
+                return;
+            }
+            if (s.c + 1 < nc && s.qx1 > (R)0) {
+                out += s.qx1;
+            }
+            if (s.c > 0 && s.xt < (R)0) {
+                out += -s.xt;
+            }
+            if (qB + 1 < nr && s.qy1 > (R)0) {
+                out += s.qy1;
+            }
+            if (qB > 0 && s.qy2 < (R)0) {
+                out += -s.qy2;
+            }
+            if (K8) {
+                if (qB + 1 < nr && s.c + 1 < nc && s.qd1_1 > (R)0) {
+                    out += s.qd1_1;
                 }
-                if (s.c == nc - 1) {
-                    s.oE0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, rden, p, ce);
+                if (qB > 0 && s.c > 0 && s.xu < (R)0) {
+                    out += -s.xu;
                 }
-                if (r == 0) {
-                    s.oN0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, rden, p, cn);
+                if (qB + 1 < nr && s.c > 0 && s.xv > (R)0) {
+                    out += s.xv;
                 }
-                if (r == nr - 1) {
-                    s.oS0 = open_flux<R>(s.z0, s.h0, s.n0, dt, den, rden, p, cs);
-                }
-                if (outA && s.outcol) {
-                    s.cap += cw + ce + cn + cs;
+                if (qB > 0 && s.c + 1 < nc && s.qd2_2 < (R)0) {
+                    out += -s.qd2_2;
                 }
             }
+            if (open_row) {
+                const OpenFaces<R> o =
+                    open_faces<R>(s.z1, s.h1, s.n1, s.m1, s.c, qB, nr, nc, dt, den, rden, p);
+                out += o.w + o.e + o.n + o.s;
+                if (outB && s.outcol) {
+                    s.cap += o.cap;
+                }
+            }
+            out = out * dt;
+            s.s1 = (s.m1 == FCA_LAND) ? limiter_s<R>(out, s.h1, A) : (R)1;
         });
+    }
 
-        /* ---- B: outflow volume and limiter of row r-1 -------------------- */
-        lp.pull_left(&L::xt, &L::qx1);
-        if (K8) {
-            lp.pull_left(&L::xu, &L::qd1_2);
-            lp.pull_left(&L::xv, &L::qd2_1);
-        }
-        if (qB >= 0 && qB < nr) {
-            lp.each([&](L &s, int) {
-                R out = (R)0;
-                if (!s.colok) {
-                    s.s1 = (R)1;
-                    return;
-                }
-                if (s.c + 1 < nc && s.qx1 > (R)0) {
-                    out += s.qx1;
-                }
-                if (s.c > 0 && s.xt < (R)0) {
-                    out += -s.xt;
-                }
-                if (qB + 1 < nr && s.qy1 > (R)0) {
-                    out += s.qy1;
-                }
-                if (qB > 0 && s.qy2 < (R)0) {
-                    out += -s.qy2;
-                }
-                if (K8) {
-                    if (qB + 1 < nr && s.c + 1 < nc && s.qd1_1 > (R)0) {
-                        out += s.qd1_1;
-                    }
-                    if (qB > 0 && s.c > 0 && s.xu < (R)0) {
-                        out += -s.xu;
-                    }
-                    if (qB + 1 < nr && s.c > 0 && s.xv > (R)0) {
-                        out += s.xv;
-                    }
-                    if (qB > 0 && s.c + 1 < nc && s.qd2_2 < (R)0) {
-                        out += -s.qd2_2;
-                    }
-                }
-                out += s.oW1 + s.oE1 + s.oN1 + s.oS1;
-                out = out * dt;
-                s.s1 = (s.m1 == FCA_LAND) ? limiter_s<R>(out, s.h1, A) : (R)1;
-            });
-        }
-
-        /* ---- C: update row r-2 ------------------------------------------- */
-        lp.pull_right(&L::xw, &L::s2);
-        lp.pull_left(&L::xy, &L::s2);
-        lp.pull_left(&L::xa, &L::qx2);
+    /* ---- C: update row r-2 --------------------------------------------- */
+    lp.pull_right(&L::xw, &L::s2);
+    lp.pull_left(&L::xy, &L::s2);
+    lp.pull_left(&L::xa, &L::qx2);
+    if (K8) {
+        lp.pull_right(&L::dz1, &L::s1); /* s of (q+1, c+1) */
+        lp.pull_left(&L::dz2, &L::s1);  /* s of (q+1, c-1) */
+        lp.pull_right(&L::dz3, &L::s3); /* s of (q-1, c+1) */
+        lp.pull_left(&L::dz4, &L::s3);  /* s of (q-1, c-1) */
+        lp.pull_left(&L::dh1, &L::qd1_3);
+        lp.pull_left(&L::dh2, &L::qd2_2);
+    }
+    if (!LANDF) {
         lp.pull_right(&L::xm3, &L::m2);
         lp.pull_left(&L::xm4, &L::m2);
         if (K8) {
-            lp.pull_right(&L::dz1, &L::s1); /* s of (q+1, c+1) */
-            lp.pull_left(&L::dz2, &L::s1);  /* s of (q+1, c-1) */
-            lp.pull_right(&L::dz3, &L::s3); /* s of (q-1, c+1) */
-            lp.pull_left(&L::dz4, &L::s3);  /* s of (q-1, c-1) */
-            lp.pull_left(&L::dh1, &L::qd1_3);
-            lp.pull_left(&L::dh2, &L::qd2_2);
             lp.pull_right(&L::dm1, &L::m1);
             lp.pull_left(&L::dm2, &L::m3);
             lp.pull_left(&L::dm3, &L::m1);
             lp.pull_right(&L::dm4, &L::m3);
         }
-        if (qC >= 0 && qC < nr) {
-            const int outC = qC >= r_out0 && qC < r_out1;
-            lp.each([&](L &s, int) {
-                R net = (R)0, q, qos, rain, hnew;
-                double sea = 0.0;
-                const int land = s.m2 == FCA_LAND;
-                if (!s.colok) {
-                    return;
+    }
+    if (qC >= 0 && qC < nr) {
+        const int outC = qC >= w.r_out0 && qC < w.r_out1;
+        const int open_row = !LANDF && p.open && (w.edge_cols || qC == 0 || qC == nr - 1);
+        lp.each([&](L &s, int) {
+            R net = (R)0, q, rain, hnew;
+            double sea = 0.0;
+            const int land = LANDF || s.m2 == FCA_LAND;
+            if (!LANDF && !s.colok) {
+                return;
+            }
+            /* E, W, S, N[, SE, NW, SW, NE] in the reference's order */
+            if (LANDF || s.c + 1 < nc) {
+                q = s.qx2 * ((s.qx2 >= (R)0) ? s.s2 : s.xw);
+                net -= q;
+                if (!LANDF && land && s.xm3 == FCA_SEA && q > (R)0) {
+                    sea += (double)dt * (double)q;
                 }
-                /* E, W, S, N[, SE, NW, SW, NE] in the reference's order */
-                if (s.c + 1 < nc) {
-                    q = s.qx2 * ((s.qx2 >= (R)0) ? s.s2 : s.xw);
+            }
+            if (LANDF || s.c > 0) {
+                q = s.xa * ((s.xa >= (R)0) ? s.xy : s.s2);
+                net += q;
+                if (!LANDF && land && s.xm4 == FCA_SEA && q < (R)0) {
+                    sea += (double)dt * (double)(-q);
+                }
+            }
+            if (LANDF || qC + 1 < nr) {
+                q = s.qy2 * ((s.qy2 >= (R)0) ? s.s2 : s.s1);
+                net -= q;
+                if (!LANDF && land && s.m1 == FCA_SEA && q > (R)0) {
+                    sea += (double)dt * (double)q;
+                }
+            }
+            if (LANDF || qC > 0) {
+                q = s.qy3 * ((s.qy3 >= (R)0) ? s.s3 : s.s2);
+                net += q;
+                if (!LANDF && land && s.m3 == FCA_SEA && q < (R)0) {
+                    sea += (double)dt * (double)(-q);
+                }
+            }
+            if (K8) {
+                if (LANDF || (qC + 1 < nr && s.c + 1 < nc)) {
+                    q = s.qd1_2 * ((s.qd1_2 >= (R)0) ? s.s2 : s.dz1);
                     net -= q;
-                    if (land && s.xm3 == FCA_SEA && q > (R)0) {
+                    if (!LANDF && land && s.dm1 == FCA_SEA && q > (R)0) {
                         sea += (double)dt * (double)q;
                     }
                 }
-                if (s.c > 0) {
-                    q = s.xa * ((s.xa >= (R)0) ? s.xy : s.s2);
+                if (LANDF || (qC > 0 && s.c > 0)) {
+                    q = s.dh1 * ((s.dh1 >= (R)0) ? s.dz4 : s.s2);
                     net += q;
-                    if (land && s.xm4 == FCA_SEA && q < (R)0) {
+                    if (!LANDF && land && s.dm2 == FCA_SEA && q < (R)0) {
                         sea += (double)dt * (double)(-q);
                     }
                 }
-                if (qC + 1 < nr) {
-                    q = s.qy2 * ((s.qy2 >= (R)0) ? s.s2 : s.s1);
+                if (LANDF || (qC + 1 < nr && s.c > 0)) {
+                    q = s.dh2 * ((s.dh2 >= (R)0) ? s.s2 : s.dz2);
                     net -= q;
-                    if (land && s.m1 == FCA_SEA && q > (R)0) {
+                    if (!LANDF && land && s.dm3 == FCA_SEA && q > (R)0) {
                         sea += (double)dt * (double)q;
                     }
                 }
-                if (qC > 0) {
-                    q = s.qy3 * ((s.qy3 >= (R)0) ? s.s3 : s.s2);
+                if (LANDF || (qC > 0 && s.c + 1 < nc)) {
+                    q = s.qd2_3 * ((s.qd2_3 >= (R)0) ? s.dz3 : s.s2);
                     net += q;
-                    if (land && s.m3 == FCA_SEA && q < (R)0) {
+                    if (!LANDF && land && s.dm4 == FCA_SEA && q < (R)0) {
                         sea += (double)dt * (double)(-q);
                     }
                 }
-                if (K8) {
-                    if (qC + 1 < nr && s.c + 1 < nc) {
-                        q = s.qd1_2 * ((s.qd1_2 >= (R)0) ? s.s2 : s.dz1);
-                        net -= q;
-                        if (land && s.dm1 == FCA_SEA && q > (R)0) {
-                            sea += (double)dt * (double)q;
-                        }
-                    }
-                    if (qC > 0 && s.c > 0) {
-                        q = s.dh1 * ((s.dh1 >= (R)0) ? s.dz4 : s.s2);
-                        net += q;
-                        if (land && s.dm2 == FCA_SEA && q < (R)0) {
-                            sea += (double)dt * (double)(-q);
-                        }
-                    }
-                    if (qC + 1 < nr && s.c > 0) {
-                        q = s.dh2 * ((s.dh2 >= (R)0) ? s.s2 : s.dz2);
-                        net -= q;
-                        if (land && s.dm3 == FCA_SEA && q > (R)0) {
-                            sea += (double)dt * (double)q;
-                        }
-                    }
-                    if (qC > 0 && s.c + 1 < nc) {
-                        q = s.qd2_3 * ((s.qd2_3 >= (R)0) ? s.dz3 : s.s2);
-                        net += q;
-                        if (land && s.dm4 == FCA_SEA && q < (R)0) {
-                            sea += (double)dt * (double)(-q);
-                        }
-                    }
-                }
-                qos = s.oW2 * s.s2 + s.oE2 * s.s2 + s.oN2 * s.s2 + s.oS2 * s.s2;
+            }
+            if (open_row) {
+                /* the open faces of row r-2 again (counted in B one row ago) */
+                const OpenFaces<R> o =
+                    open_faces<R>(s.z2, s.h2, s.n2, s.m2, s.c, qC, nr, nc, dt, den, rden, p);
+                const R qos = o.w * s.s2 + o.e * s.s2 + o.n * s.s2 + o.s * s.s2;
                 net -= qos;
                 if (outC && s.outcol) {
-                    s.v_sea += sea;
                     s.v_bnd += (double)dt * (double)qos;
                 }
-                if (!land) {
-                    s.hn2 = s.h2;
-                    return;
+            }
+            if (!LANDF && outC && s.outcol) {
+                s.v_sea += sea;
+            }
+            if (!land) {
+                s.hn2 = s.h2;
+                return;
+            }
+            {
+                double clamp_here = 0.0;
+                rain = s.pcr * sv.rain_f;
+                hnew = s.h2 + (dt * div_c(net, A, p.rA, p.fA) + rain);
+                if (hnew < (R)0) {
+                    clamp_here = (double)(-hnew) * (double)A;
+                    hnew = (R)0;
                 }
-                {
-                    size_t i = (size_t)qC * (size_t)nc + (size_t)s.c;
-                    double clamp_here = 0.0;
-                    rain = s.pcr * sv.rain_f;
-                    hnew = s.h2 + (dt * div_c(net, A, p.rA, p.fA) + rain);
-                    if (hnew < (R)0) {
-                        clamp_here = (double)(-hnew) * (double)A;
-                        hnew = (R)0;
-                    }
-                    s.hn2 = hnew;
-                    if (outC && s.outcol) {
-                        s.v_rain += (double)A * (double)rain;
-                        s.clamp += clamp_here;
-                        a.h_out[i] = hnew;
-                        if (hnew > s.phm) {
-                            a.hmax[i] = hnew;
-                            a.tmax[i] = sv.tmax_val;
-                        }
-                    }
-                }
-            });
-        }
-
-        /* prefetch cr and hmax of row r-1, which the next iteration updates */
-        lp.each([&](L &s, int) {
-            const int q = r - 1;
-            if (s.colok && q >= 0 && q < nr && s.m1 == FCA_LAND) {
-                const size_t i = (size_t)q * (size_t)nc + (size_t)s.c;
-                s.pcr = a.cr[i];
-                if (s.outcol && q >= r_out0 && q < r_out1) {
-                    s.phm = a.hmax[i];
+                s.hn2 = hnew;
+                if (outC && s.outcol) {
+                    s.v_rain += (double)A * (double)rain;
+                    s.clamp += clamp_here;
+                    s.stq = (hnew > s.phm) ? 3 : 1;
                 }
             }
         });
+    }
 
-        /* ---- D: next step's stable time step on row r-3 ------------------ */
-        if (sv.need_cand && qD >= r_out0 && qD < r_out1 && qD < nr) {
-            lp.pull_right(&L::xz, &L::z3);
-            lp.pull_right(&L::xh, &L::hn3);
+    /* ---- D: next step's stable time step on row r-3 -------------------- */
+    if (sv.need_cand && qD >= w.r_out0 && qD < w.r_out1 && qD < nr) {
+        lp.pull_right(&L::xz, &L::z3);
+        lp.pull_right(&L::xh, &L::hn3);
+        lp.pull_left(&L::xq, &L::z3);
+        lp.pull_left(&L::xr, &L::hn3);
+        if (!LANDF) {
             lp.pull_right(&L::xm, &L::m3);
-            lp.pull_left(&L::xq, &L::z3);
-            lp.pull_left(&L::xr, &L::hn3);
             lp.pull_left(&L::xm2, &L::m3);
-            if (K8) {
-                lp.pull_right(&L::dz1, &L::z2);
-                lp.pull_right(&L::dh1, &L::hn2);
+        }
+        if (K8) {
+            lp.pull_right(&L::dz1, &L::z2);
+            lp.pull_right(&L::dh1, &L::hn2);
+            lp.pull_left(&L::dz2, &L::z2);
+            lp.pull_left(&L::dh2, &L::hn2);
+            lp.pull_right(&L::dz3, &L::z4);
+            lp.pull_right(&L::dh3, &L::hn4);
+            lp.pull_left(&L::dz4, &L::z4);
+            lp.pull_left(&L::dh4, &L::hn4);
+            if (!LANDF) {
                 lp.pull_right(&L::dm1, &L::m2);
-                lp.pull_left(&L::dz2, &L::z2);
-                lp.pull_left(&L::dh2, &L::hn2);
                 lp.pull_left(&L::dm2, &L::m2);
-                lp.pull_right(&L::dz3, &L::z4);
-                lp.pull_right(&L::dh3, &L::hn4);
                 lp.pull_right(&L::dm3, &L::m4);
-                lp.pull_left(&L::dz4, &L::z4);
-                lp.pull_left(&L::dh4, &L::hn4);
                 lp.pull_left(&L::dm4, &L::m4);
             }
-            lp.each([&](L &s, int) {
-                R la, ax = (R)0, smax, v;
-                if (!s.outcol || s.m3 != FCA_LAND || s.hn3 < p.tau_h) {
-                    return;
-                }
-                la = s.z3 + s.hn3;
-                /* largest |level difference| per face length: max_k RN(a_k/d)
-                 * = RN(max_k a_k / d) as rounding is monotonic, so one
-                 * division per length replaces one per neighbour */
+        }
+        lp.each([&](L &s, int) {
+            R la, ax = (R)0, ad = (R)0, s_hi;
+            s.dneed = 0;
+            if (!s.outcol || (!LANDF && s.m3 != FCA_LAND) || s.hn3 < p.tau_h) {
+                return;
+            }
+            la = s.z3 + s.hn3;
+            /* largest |level difference| per face length: max_k RN(a_k/d)
+             * = RN(max_k a_k / d) as rounding is monotonic, so one
+             * division per length replaces one per neighbour. The LAND form
+             * takes fmax of magnitudes: it differs from the general form
+             * only in the sign of a zero maximum, and a zero slope gives the
+             * floor speed either way. */
 #define FCA_LEVEL_DIFF(cond, m_, z_, h_, acc_)                                \
-    if ((cond) && (m_) != FCA_WALL) {                                           \
+    if (LANDF) {                                                                \
+        acc_ = fmax_r(acc_, fabs_r(la - ((z_) + (h_))));                        \
+    } else if ((cond) && (m_) != FCA_WALL) {                                    \
         R dl_ = la - level_of<R>((m_), (z_), (h_), p);                          \
         if (dl_ < (R)0) {                                                       \
             dl_ = -dl_;                                                         \
@@ -426,39 +523,191 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             acc_ = dl_;                                                         \
         }                                                                       \
     }
-                FCA_LEVEL_DIFF(s.c + 1 < nc, s.xm, s.xz, s.xh, ax)
-                FCA_LEVEL_DIFF(s.c > 0, s.xm2, s.xq, s.xr, ax)
-                FCA_LEVEL_DIFF(qD + 1 < nr, s.m2, s.z2, s.hn2, ax)
-                FCA_LEVEL_DIFF(qD > 0, s.m4, s.z4, s.hn4, ax)
-                if (p.open && (s.c == 0 || s.c == nc - 1 || qD == 0 || qD == nr - 1)) {
-                    R dl_ = la - (s.z3 - p.S_bnd * p.dx);
-                    if (dl_ < (R)0) {
-                        dl_ = -dl_;
-                    }
-                    if (dl_ > ax) {
-                        ax = dl_;
-                    }
+            FCA_LEVEL_DIFF(s.c + 1 < nc, s.xm, s.xz, s.xh, ax)
+            FCA_LEVEL_DIFF(s.c > 0, s.xm2, s.xq, s.xr, ax)
+            FCA_LEVEL_DIFF(qD + 1 < nr, s.m2, s.z2, s.hn2, ax)
+            FCA_LEVEL_DIFF(qD > 0, s.m4, s.z4, s.hn4, ax)
+            if (!LANDF && p.open && (s.c == 0 || s.c == nc - 1 || qD == 0 || qD == nr - 1)) {
+                R dl_ = la - (s.z3 - p.S_bnd * p.dx);
+                if (dl_ < (R)0) {
+                    dl_ = -dl_;
                 }
-                smax = div_c(ax, p.dx, p.rdx, p.fdx);
-                if (K8) {
-                    R ad = (R)0, sd;
-                    FCA_LEVEL_DIFF(qD + 1 < nr && s.c + 1 < nc, s.dm1, s.dz1, s.dh1, ad)
-                    FCA_LEVEL_DIFF(qD + 1 < nr && s.c > 0, s.dm2, s.dz2, s.dh2, ad)
-                    FCA_LEVEL_DIFF(qD > 0 && s.c + 1 < nc, s.dm3, s.dz3, s.dh3, ad)
-                    FCA_LEVEL_DIFF(qD > 0 && s.c > 0, s.dm4, s.dz4, s.dh4, ad)
-                    sd = div_c(ad, p.ddiag, p.rddiag, p.fddiag);
-                    if (sd > smax) {
-                        smax = sd;
-                    }
+                if (dl_ > ax) {
+                    ax = dl_;
                 }
+            }
+            if (K8) {
+                FCA_LEVEL_DIFF(qD + 1 < nr && s.c + 1 < nc, s.dm1, s.dz1, s.dh1, ad)
+                FCA_LEVEL_DIFF(qD + 1 < nr && s.c > 0, s.dm2, s.dz2, s.dh2, ad)
+                FCA_LEVEL_DIFF(qD > 0 && s.c + 1 < nc, s.dm3, s.dz3, s.dh3, ad)
+                FCA_LEVEL_DIFF(qD > 0 && s.c > 0, s.dm4, s.dz4, s.dh4, ad)
+            }
 #undef FCA_LEVEL_DIFF
-                /* dt_a = alpha*dx / v is non-increasing in v, so the lane keeps
-                 * the largest v and divides once at the end of the strip */
-                v = cell_speed<R>(s.hn3, s.n3, smax, p);
-                if (v > s.vmax) {
-                    s.vmax = v;
+            s.dax = ax, s.dad = ad;
+            /* RN(ax*RN(1/dx)) is within 2^-23 of ax/dx, so after the factor
+             * (1+2^-20) it bounds RN(ax/dx) from above */
+            s_hi = ax * p.rdx;
+            if (K8) {
+                const R sd = ad * p.rddiag;
+                s_hi = (sd > s_hi) ? sd : s_hi;
+            }
+            s_hi = s_hi * ((R)1 + (R)0x1p-20);
+            s.dneed =
+                speed_may_exceed<R>(s.hn3, s.n3, s_hi, ax == (R)0 && ad == (R)0, s.vthr, p.g);
+        });
+        if (lp.any(&L::dneed)) {
+            lp.each([&](L &s, int) {
+                if (s.dneed) {
+                    R smax = div_c(s.dax, p.dx, p.rdx, p.fdx), v;
+                    if (K8) {
+                        const R sd = div_c(s.dad, p.ddiag, p.rddiag, p.fddiag);
+                        if (sd > smax) {
+                            smax = sd;
+                        }
+                    }
+                    v = cell_speed<R>(s.hn3, s.n3, smax, p);
+                    if (v > s.vmax) {
+                        s.vmax = v;
+                    }
                 }
             });
+            lp.wave_max_to(&L::vthr, &L::vmax);
+        }
+    }
+}
+
+/* Walks one strip for one wavefront. strip covers output rows
+ * [r_out0, r_out1) and columns [c_first + FCA_HALO, c_first + FCA_HALO + 58). */
+template <class R, int K8, class LP>
+FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int c_first)
+{
+    typedef WalkLane<R> L;
+    const int nr = a.nr, nc = a.nc;
+    const int rb = (r_out0 - 3 > 0) ? r_out0 - 3 : 0;
+    const int re = (r_out1 + 2 < nr + 2) ? r_out1 + 2 : nr + 2; /* inclusive */
+    /* every lane inside the grid and none on its first or last column */
+    const int cols_inner = c_first >= 1 && c_first + FCA_WAVE - 1 <= nc - 2;
+    WalkRow w;
+    unsigned land_rows; /* bit k: row r-k is LAND on every lane */
+    unsigned void_rows; /* bit k: row r-k has no LAND lane */
+    int r;
+
+    w.r_out0 = r_out0, w.r_out1 = r_out1, w.c_first = c_first;
+    w.edge_cols = c_first <= 0 || c_first + FCA_WAVE - 1 >= nc - 1;
+    lp.each([&](L &s, int lane) {
+        s.c = c_first + lane;
+        s.colok = s.c >= 0 && s.c < nc;
+        s.outcol = s.colok && lane >= FCA_HALO && lane < FCA_WAVE - FCA_HALO;
+        s.boff = (uint32_t)s.c * (uint32_t)sizeof(R);
+        s.z0 = s.z1 = s.z2 = s.z3 = s.z4 = (R)0;
+        s.m0 = s.m1 = s.m2 = s.m3 = s.m4 = FCA_NONE;
+        s.h0 = s.h1 = s.h2 = (R)0;
+        s.n0 = s.n1 = s.n2 = s.n3 = (R)0;
+        s.lv0 = s.lv1 = (R)0;
+        s.hn2 = s.hn3 = s.hn4 = (R)0;
+        s.qx0 = s.qx1 = s.qx2 = s.qy1 = s.qy2 = s.qy3 = (R)0;
+        s.qd1_1 = s.qd1_2 = s.qd1_3 = s.qd2_1 = s.qd2_2 = s.qd2_3 = (R)0;
+        s.s1 = s.s2 = s.s3 = (R)1;
+        s.pcr = s.phm = s.pcr1 = s.phm1 = (R)0;
+        s.stq = 0;
+        s.pz = s.ph = s.pn = (R)0;
+        s.pm1 = s.pm2 = FCA_NONE;
+        if (s.colok && rb < nr) {
+            const size_t ro = (size_t)rb * (size_t)nc;
+            s.pm1 = a.mask[ro + (uint32_t)s.c];
+            if (s.pm1 == FCA_LAND) {
+                s.pz = ld_row(a.z + ro, s.boff);
+                s.ph = ld_row(a.h_in + ro, s.boff);
+                s.pn = ld_row(a.n + ro, s.boff);
+            }
+        }
+        if (s.colok && rb + 1 < nr) {
+            s.pm2 = a.mask[(size_t)(rb + 1) * (size_t)nc + (uint32_t)s.c];
+        }
+    });
+    /* rows before rb are not loaded; the ring holds them as outside the grid */
+    land_rows = 0u;
+    void_rows = ~0u;
+
+    for (r = rb; r <= re; r++) {
+        /* take row r (prefetched), prefetch z/h/n of row r+1 and the mask of
+         * row r+2 */
+        /* Every memory access of a row is issued here, at the top: the
+         * loads are consumed one row later, so each has a whole row of work
+         * to arrive in, and the stores of row r-3 (computed one row ago)
+         * follow the wait for the previous row's loads, so no wait ever
+         * covers a store that was just issued. */
+        {
+            const int ok1 = r + 1 < nr, ok2 = r + 2 < nr;
+            const int q = r - 1, okq = q >= 0 && q < nr;
+            const int outq = q >= r_out0 && q < r_out1;
+            const size_t ro1 = (size_t)(r + 1) * (size_t)nc, ro2 = (size_t)(r + 2) * (size_t)nc;
+            const size_t roq = (size_t)(okq ? q : 0) * (size_t)nc;
+            const R *zr = a.z + ro1, *hr = a.h_in + ro1, *nrw = a.n + ro1;
+            const R *crr = a.cr + roq, *hmr = a.hmax + roq;
+            const uint8_t *mr = a.mask + ro2;
+            const int qs = r - 3, oks = qs >= 0 && qs < nr;
+            const size_t ros = (size_t)(oks ? qs : 0) * (size_t)nc;
+            R *hos = a.h_out + ros, *hms = a.hmax + ros, *tms = a.tmax + ros;
+            const R tval = a.sv.tmax_val;
+            lp.each([&](L &s, int) {
+                s.m0 = s.pm1;
+                if (s.m0 == FCA_LAND) {
+                    s.z0 = s.pz, s.h0 = s.ph, s.n0 = s.pn;
+                }
+                s.lv0 = s.z0 + s.h0;
+            });
+            /* the wait for the previous row's loads goes here, before this
+             * row's stores are issued */
+            lp.loads_settled();
+            lp.each([&](L &s, int) {
+                if (s.stq) {
+                    st_row(hos, s.boff, s.hn3);
+                    if (s.stq & 2) {
+                        st_row(hms, s.boff, s.hn3);
+                        st_row(tms, s.boff, tval);
+                    }
+                    s.stq = 0;
+                }
+                s.pm1 = s.pm2;
+                if (s.colok && ok1 && s.pm1 == FCA_LAND) {
+                    s.pz = ld_row(zr, s.boff), s.ph = ld_row(hr, s.boff);
+                    s.pn = ld_row(nrw, s.boff);
+                }
+                s.pm2 = (s.colok && ok2) ? (int)mr[(uint32_t)s.c] : FCA_NONE;
+                /* cr and hmax of row r-1 (m1), updated one row from now;
+                 * hmax is read before that update writes it */
+                if (s.colok && okq && s.m1 == FCA_LAND) {
+                    s.pcr1 = ld_row(crr, s.boff);
+                    if (s.outcol && outq) {
+                        s.phm1 = ld_row(hmr, s.boff);
+                    }
+                }
+            });
+        }
+        /* the row forms from row r's mask, fetched two rows ago, so that no
+         * load of this row is waited for here */
+        land_rows = (land_rows << 1) | (lp.all_land(&L::m0) ? 1u : 0u);
+        void_rows = (void_rows << 1) | (lp.no_land(&L::m0) ? 1u : 0u);
+        w.r = r;
+        if (a.p.fast && cols_inner && (land_rows & 0x1Fu) == 0x1Fu && r - 4 >= 1 &&
+            r <= nr - 2) {
+            lp.count_form(1);
+            walk_row<R, K8, 1>(lp, a, w);
+        } else if ((void_rows & 0xFu) == 0xFu) {
+            /* rows r-3..r hold no LAND cell: every face touching rows r-1
+             * and r is dead, the limiter of row r-1 is 1, row r-2 keeps its
+             * depth and row r-3 has no time step (the general form's values
+             * for non-LAND cells) */
+            lp.count_form(2);
+            lp.each([&](L &s, int) {
+                s.qx0 = s.qy1 = s.qd1_1 = s.qd2_1 = (R)0;
+                s.s1 = (R)1;
+                s.hn2 = s.h2;
+            });
+        } else {
+            lp.count_form(0);
+            walk_row<R, K8, 0>(lp, a, w);
         }
 
         /* ---- rotate the ring ----------------------------------------------- */
@@ -467,14 +716,30 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             s.m4 = s.m3, s.m3 = s.m2, s.m2 = s.m1, s.m1 = s.m0;
             s.h2 = s.h1, s.h1 = s.h0;
             s.n3 = s.n2, s.n2 = s.n1, s.n1 = s.n0;
+            s.lv1 = s.lv0;
             s.hn4 = s.hn3, s.hn3 = s.hn2;
             s.qx2 = s.qx1, s.qx1 = s.qx0;
             s.qy3 = s.qy2, s.qy2 = s.qy1;
             s.qd1_3 = s.qd1_2, s.qd1_2 = s.qd1_1;
             s.qd2_3 = s.qd2_2, s.qd2_2 = s.qd2_1;
-            s.oW2 = s.oW1, s.oE2 = s.oE1, s.oN2 = s.oN1, s.oS2 = s.oS1;
-            s.oW1 = s.oW0, s.oE1 = s.oE0, s.oN1 = s.oN0, s.oS1 = s.oS0;
             s.s3 = s.s2, s.s2 = s.s1;
+            s.pcr = s.pcr1, s.phm = s.phm1;
+        });
+    }
+    /* the last row's update is never an output row (re = r_out1 + 2), so no
+     * deferred store is left; flush one all the same if the bounds change */
+    if (lp.any(&L::stq)) {
+        const size_t ros = (size_t)(re - 2) * (size_t)nc;
+        R *hos = a.h_out + ros, *hms = a.hmax + ros, *tms = a.tmax + ros;
+        lp.each([&](L &s, int) {
+            if (s.stq) {
+                st_row(hos, s.boff, s.hn3);
+                if (s.stq & 2) {
+                    st_row(hms, s.boff, s.hn3);
+                    st_row(tms, s.boff, a.sv.tmax_val);
+                }
+                s.stq = 0;
+            }
         });
     }
 }
@@ -488,6 +753,7 @@ FCA_HD void walk_acc_init(LP &lp)
         s.cap = 0;
         s.dtc = (R)INFINITY;
         s.vmax = (R)0;
+        s.vthr = (R)0;
     });
 }
 

@@ -64,6 +64,8 @@ struct FcaHipSim {
     real *h_stage;         /* pinned staging for forcing reloads */
     FcaRunCfg *cfg;        /* copy of the run configuration */
     double t_h2d, t_io;
+    int n_ok;              /* every LAND roughness in the LAND form's range */
+    int no_land_form;      /* FCA_HIP_NO_LAND_FORM was asked for */
 };
 
 static void dev_free(FcaDev *d)
@@ -153,13 +155,19 @@ static int div_const_exact(double dd)
 #endif
 }
 
-/* The divisors of the step: dx, (real)sqrt(2.0)*dx (physics.h:45-48), A. */
-static void set_div_flags(FcaDev *d)
+/* The divisors of the step: dx, (real)sqrt(2.0)*dx (physics.h:45-48), A;
+ * and whether the fused walk may take its LAND form (n_ok: every LAND
+ * roughness in range, from the static upload). */
+static void set_div_flags(FcaDev *d, int n_ok, int no_land_form)
 {
     const real dx = d->phys.dx, ddiag = (real)sqrt(2.0) * dx;
     d->fdiv[0] = div_const_exact((double)dx);
     d->fdiv[1] = d->phys.K == 8 ? div_const_exact((double)ddiag) : 0;
     d->fdiv[2] = div_const_exact((double)d->phys.A);
+    d->land_form = !no_land_form && n_ok && fca_land_phys_ok(&d->phys);
+#ifndef FCA_DOUBLE
+    d->land_form = d->land_form && d->fdiv[0] && d->fdiv[2] && (d->phys.K != 8 || d->fdiv[1]);
+#endif
 }
 
 static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
@@ -171,7 +179,10 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
     d->nc = g->ncols;
     d->K = cfg->phys.K;
     d->kernel = o ? o->kernel : FCA_HIP_FUSED;
-    d->strip_rows = (o && o->strip_rows > 0) ? o->strip_rows : 256;
+    /* 96 rows: short enough that a 16384-column grid gives several waves of
+     * wavefronts per CU (no long tail), long enough that the 5 halo rows a
+     * strip re-walks cost 5% */
+    d->strip_rows = (o && o->strip_rows > 0) ? o->strip_rows : 96;
     d->rec_cap = (o && o->record_steps > 0) ? o->record_steps : 0;
     d->phys = cfg->phys;
     d->ctl.T_sim = (double)cfg->T_sim;
@@ -182,7 +193,8 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
     d->ctl.steps = cfg->steps;
     d->ctl.U = cfg->phys.U;
     d->ctl.n_force_days = cfg->n_force_days;
-    set_div_flags(d);
+    s->no_land_form = o && (o->flags & FCA_HIP_NO_LAND_FORM);
+    set_div_flags(d, 0, s->no_land_form);
     d->max_blocks = (o && o->max_blocks > 0) ? o->max_blocks : 0;
     fca_k_geometry(d);
     s->N = N;
@@ -234,6 +246,8 @@ static int sim_upload_static(FcaHipSim *s, const FcaGrid *g)
     HIP_OR(hipMemcpy(d->C, g->C, B, hipMemcpyHostToDevice), -2);
     HIP_OR(hipMemcpy(d->mask, g->mask, s->N, hipMemcpyHostToDevice), -2);
     s->t_h2d += now_s() - t0;
+    s->n_ok = fca_land_n_ok(g->n, g->mask, s->N);
+    set_div_flags(d, s->n_ok, s->no_land_form);
     return 0;
 }
 
@@ -675,16 +689,16 @@ void fca_hip_device_name(char *name, size_t n)
         buf[0] = 0;
     }
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
-        snprintf(arch, sizeof(arch), "%s", prop.gcnArchName);
+        snprintf(arch, sizeof(arch), "%.63s", prop.gcnArchName);
         if (!buf[0]) {
-            snprintf(buf, sizeof(buf), "%s", prop.name);
+            snprintf(buf, sizeof(buf), "%.255s", prop.name);
         }
     }
     if (!buf[0]) {
         snprintf(buf, sizeof(buf), "AMD GPU");
     }
     if (arch[0]) {
-        snprintf(name, n, "%s (%s)", buf, arch);
+        snprintf(name, n, "%.255s (%.63s)", buf, arch);
     } else {
         snprintf(name, n, "%s", buf);
     }
@@ -774,7 +788,7 @@ int fca_hip_ens_run(FcaHipEns *e, FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc,
     /* per-scenario configuration: forcing shape, horizon, step count */
     memcpy(s->cfg, cfg, sizeof(FcaRunCfg));
     s->d.phys = cfg->phys;
-    set_div_flags(&s->d);
+    set_div_flags(&s->d, s->n_ok, s->no_land_form);
     s->d.ctl.T_sim = (double)cfg->T_sim;
     s->d.ctl.out_every = (double)cfg->out_every;
     s->d.ctl.D_h = (double)cfg->D_h;

@@ -243,13 +243,17 @@ enum { FCA_HIP_FUSED = 0, FCA_HIP_SPLIT = 1 };
 /* Per-session tuning; zero fields take the defaults. */
 typedef struct FcaHipOpts {
     int kernel;        /* FCA_HIP_FUSED or FCA_HIP_SPLIT */
-    int strip_rows;    /* fused: rows walked by one wavefront (default 256) */
+    int strip_rows;    /* fused: rows walked by one wavefront (default 96) */
     int graph_batch;   /* steps per captured hipGraph replay (default 32; 1 = no graph) */
     int record_steps;  /* > 0: keep a per-step log of the first record_steps steps */
     int max_blocks;    /* fused: cap on workgroups, each then walking several strips
                         * (default 0: one strip per wavefront, hardware-dispatched) */
-    int reserved[3];
+    int flags;         /* FCA_HIP_NO_LAND_FORM: the fused walk never takes its LAND
+                        * form (the same results; for comparisons) */
+    int reserved[2];
 } FcaHipOpts;
+
+enum { FCA_HIP_NO_LAND_FORM = 1 };
 
 /* One executed step, as the controller saw it (fca_step, model.c:363-382). */
 typedef struct FcaHipStepRec {

@@ -85,7 +85,10 @@ def emu_lib(prec):
     return L
 
 
-def emu_run(grid, cfg, acc, prec, strip_rows=0, cap=100000):
+def emu_run(grid, cfg, acc, prec, strip_rows=0, cap=100000, land_form=True, forms=None):
+    """Runs the emulated fused walk; land_form=False keeps every row in the
+    general form. forms, if a list, receives the rows walked in the general,
+    the LAND and the no-LAND form."""
     import ctypes as C
 
     a = A.abi(prec)
@@ -93,6 +96,14 @@ def emu_run(grid, cfg, acc, prec, strip_rows=0, cap=100000):
     g = grid.fca_grid(a)
     rec = (a.FcaHipStepRec * cap)()
     n = C.c_int()
-    assert L.emu_run(C.byref(g), C.byref(cfg), C.byref(acc), strip_rows, rec, cap,
-                     C.byref(n)) == 0
+    L.emu_set_land_form(-1 if land_form else 0)
+    try:
+        assert L.emu_run(C.byref(g), C.byref(cfg), C.byref(acc), strip_rows, rec, cap,
+                         C.byref(n)) == 0
+    finally:
+        L.emu_set_land_form(-1)
+    if forms is not None:
+        gen, land, noland = C.c_long(), C.c_long(), C.c_long()
+        L.emu_form_rows(C.byref(gen), C.byref(land), C.byref(noland))
+        forms[:] = [gen.value, land.value, noland.value]
     return acc, list(rec)[:min(n.value, cap)]

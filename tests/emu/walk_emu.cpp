@@ -48,7 +48,68 @@ struct CpuLanes {
             s[l].*dst = tmp[l];
         }
     }
+    long form_rows[3];
+    void count_form(int form)
+    {
+        form_rows[form]++;
+    }
+    void loads_settled() {}
+    bool no_land(int WalkLane<R>::*m)
+    {
+        for (int l = 0; l < FCA_WAVE; l++) {
+            if (s[l].*m == FCA_LAND) {
+                return false;
+            }
+        }
+        return true;
+    }
+    bool all_land(int WalkLane<R>::*m)
+    {
+        for (int l = 0; l < FCA_WAVE; l++) {
+            if (s[l].*m != FCA_LAND) {
+                return false;
+            }
+        }
+        return true;
+    }
+    bool any(int WalkLane<R>::*f)
+    {
+        for (int l = 0; l < FCA_WAVE; l++) {
+            if (s[l].*f) {
+                return true;
+            }
+        }
+        return false;
+    }
+    void wave_max_to(R WalkLane<R>::*dst, R WalkLane<R>::*src)
+    {
+        R v = s[0].*src;
+        for (int l = 1; l < FCA_WAVE; l++) {
+            v = (s[l].*src > v) ? s[l].*src : v;
+        }
+        for (int l = 0; l < FCA_WAVE; l++) {
+            s[l].*dst = v;
+        }
+    }
 };
+
+/* -1: the LAND form where the data allow it (as on the device); 0: never
+ * (tests compare both forms). */
+static int emu_land_form = -1;
+extern "C" void emu_set_land_form(int v)
+{
+    emu_land_form = v;
+}
+
+/* Rows walked in the general, the LAND and the no-LAND form by the last
+ * emu_run. */
+static long emu_rows[3];
+extern "C" void emu_form_rows(long *general, long *land, long *noland)
+{
+    *general = emu_rows[0];
+    *land = emu_rows[1];
+    *noland = emu_rows[2];
+}
 
 static int load_day(const char *path, R *dst, size_t n, R lambda)
 {
@@ -82,7 +143,7 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
 {
     const int nr = g->nrows, nc = g->ncols, K8 = cfg->phys.K == 8;
     const size_t N = (size_t)nr * (size_t)nc;
-    const PhysK<R> p = make_phys<R>(cfg->phys);
+    PhysK<R> p = make_phys<R>(cfg->phys);
     FcaCtlParams ctl;
     FcaDevCtrl c, nx;
     R *h[2], *cr, cand = (R)INFINITY;
@@ -126,6 +187,9 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
     memset(&c, 0, sizeof(c));
     c.t = acc->t, c.step = acc->step, c.force_day = acc->force_day;
     c.state = FCA_CTL_RUN, c.dt_base = (double)acc->dt_base;
+    /* the walk's LAND form depends on the data only; host arithmetic needs
+     * none of the device's division flags */
+    p.fast = emu_land_form == 0 ? 0 : fca_land_form_ok(&cfg->phys, g->n, g->mask, N);
     ngroups = (nc + FCA_OUTC - 1) / FCA_OUTC;
     nstrips = (nr + strip_rows - 1) / strip_rows;
 
@@ -174,6 +238,9 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
             wa.nr = nr, wa.nc = nc, wa.strip_rows = strip_rows, wa.p = p, wa.sv = sv;
             for (int t = 0; t < ngroups * nstrips; t++) {
                 static CpuLanes lp;
+                if (t == 0 && c.step == acc0.step) {
+                    lp.form_rows[0] = lp.form_rows[1] = lp.form_rows[2] = 0;
+                }
                 const int strip = t / ngroups, grp = t % ngroups;
                 const int r0 = strip * strip_rows;
                 const int r1 = (r0 + strip_rows < nr) ? r0 + strip_rows : nr;
@@ -184,6 +251,8 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
                     walk_strip<R, 0>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
                 }
                 walk_acc_finish<R>(lp, p);
+                emu_rows[0] = lp.form_rows[0], emu_rows[1] = lp.form_rows[1];
+                emu_rows[2] = lp.form_rows[2];
                 for (int l = 0; l < FCA_WAVE; l++) {
                     acc->V_rain += lp.s[l].v_rain;
                     acc->clamp_err += lp.s[l].clamp;

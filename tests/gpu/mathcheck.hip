@@ -1,14 +1,21 @@
 /*
  * mathcheck.hip -- test-only kernels that evaluate the device scalar math of
- * fca_device.h over ranges of float bit patterns, so the GPU suite can compare
- * every input against libm (tests/test_gpu_math.py). Never part of the product.
+ * fca_device.h and fca_walk.h over ranges of float bit patterns, so the GPU
+ * suite can compare every input against libm or check a property on every
+ * input (tests/test_gpu_math.py). Never part of the product.
  */
 #include <hip/hip_runtime.h>
 
-#include "fca_device.h"
+#include "fca_walk.h"
 
-__global__ void k_eval(int kind, uint32_t first, uint32_t count, float g, float d, float rd,
-                       int dok, float *out)
+/* Parameters of one evaluation; x is the float whose bit pattern is the
+ * work-item's index. */
+struct McArgs {
+    float g, d, rd, e, f;
+    int dok, k;
+};
+
+__global__ void k_eval(int kind, uint32_t first, uint32_t count, McArgs a, float *out)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) {
@@ -16,12 +23,14 @@ __global__ void k_eval(int kind, uint32_t first, uint32_t count, float g, float 
     }
     uint32_t b = first + (uint32_t)i;
     float x = __uint_as_float(b), y;
+    PhysK<float> p;
+    p.g = a.g;
     switch (kind) {
     case 0: y = cbrt_sq(x); break;
     case 1: y = sqrt_via_double(x); break;
-    case 2: y = crit_speed(g, x); break;
+    case 2: y = crit_speed(a.g, x); break;
     case 3: y = rcp_cr(x); break;
-    case 4: y = div_c(x, d, rd, dok); break;
+    case 4: y = div_c(x, a.d, a.rd, a.dok); break;
 #if defined(__HIP_DEVICE_COMPILE__)
     case 6: { /* relative error of the Newton value inside the fast domain */
         double sq = (double)x * (double)x, y1 = cbrt_sq_newton(x, sq), z = cbrt(sq);
@@ -29,14 +38,33 @@ __global__ void k_eval(int kind, uint32_t first, uint32_t count, float g, float 
         break;
     }
 #endif
+    case 7: y = sqrt_in_range(x); break;
+    case 8: y = rcp_in_range(x); break;
+    case 9: y = div_c_big(x, a.d, a.rd); break;
+    case 10: { /* crit_cap near the critical speed: v = vcr stepped by k ulps */
+        const float v = f32_step(crit_speed(a.g, x), a.k);
+        y = crit_cap<float>(v, x, p);
+        break;
+    }
+    case 11: { /* stage D bound: 1 where it would skip a cell whose exact
+                * speed exceeds thr = (exact speed stepped by k ulps) */
+        const float h = x, n = a.e, ax = a.f;
+        const float s_hi = (ax * a.rd) * (1.0f + 0x1p-20f);
+        const float smax = div_c(ax, a.d, a.rd, a.dok);
+        const float v = cell_speed<float>(h, n, smax, p);
+        const float thr = f32_step(v, a.k);
+        const int need = speed_may_exceed<float>(h, n, s_hi, ax == 0.0f, thr, a.g);
+        y = (!need && v > thr) ? 1.0f : (need ? 0.5f : 0.0f);
+        break;
+    }
+    case 12: y = div_step_cr(x, a.d, a.rd); break;
     default: y = sqrt_rough(x); break;
     }
     out[i] = y;
 }
 
 /* Evaluates kind over [first, first+count) into host out. Returns 0 or a HIP error. */
-extern "C" int mc_eval(int kind, uint32_t first, uint32_t count, float g, float d, float rd,
-                       int dok, float *out)
+extern "C" int mc_eval(int kind, uint32_t first, uint32_t count, const McArgs *a, float *out)
 {
     float *dv = nullptr;
     hipError_t e = hipMalloc((void **)&dv, (size_t)count * sizeof(float));
@@ -44,7 +72,7 @@ extern "C" int mc_eval(int kind, uint32_t first, uint32_t count, float g, float 
         return (int)e;
     }
     hipLaunchKernelGGL(k_eval, dim3((count + 255) / 256), dim3(256), 0, 0, kind, first, count,
-                       g, d, rd, dok, dv);
+                       *a, dv);
     e = hipGetLastError();
     if (e == hipSuccess) {
         e = hipMemcpy(out, dv, (size_t)count * sizeof(float), hipMemcpyDeviceToHost);

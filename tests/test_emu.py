@@ -30,3 +30,59 @@ def test_emulated_walk_matches_oracle(prec, K, bnd, shape):
     assert [r.dt for r in log_o] == [r.dt for r in log_e]
     assert acc_o.cap_active == acc_e.cap_active and acc_o.cap_total == acc_e.cap_total
     assert acc_o.V_rain == pytest.approx(acc_e.V_rain, rel=1e-12)
+
+
+LAND_SHAPES = [(70, 250, 32), (45, 180, 9)]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("K", [4, 8])
+@pytest.mark.parametrize("bnd", [A.FCA_BND_OPEN, A.FCA_BND_CLOSED])
+@pytest.mark.parametrize("shape", LAND_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}s{s[2]}")
+def test_land_form_matches_oracle(prec, K, bnd, shape):
+    """Mostly-LAND grids wide enough for inner column groups, so that rows take
+    the walk's LAND form next to general rows (coast, walls, edges); the
+    result is the oracle's bit for bit, and the same with the LAND form off."""
+    nr, nc, strip = shape
+    a, g, cfg = make_case(nr, nc, prec=prec, K=K, boundary=bnd, T_sim=1800.0, land_frac=0.93,
+                          seed=nr + nc, h0=0.08)
+    rng = np.random.default_rng(nc)
+    w = (rng.random(g.shape) < 0.002) & (g.mask == A.FCA_LAND)
+    g.mask[w] = A.FCA_WALL
+    o = Oracle(prec)
+    go, ge, gg = g.copy(), g.copy(), g.copy()
+    acc_o, log_o = o.run(go, cfg, log_cap=100000)
+    forms = []
+    acc_e, log_e = emu_run(ge, cfg, o.acc_init(ge, cfg), prec, strip_rows=strip, forms=forms)
+    assert forms[1] > 0 and forms[0] > 0, forms
+    acc_g, log_g = emu_run(gg, cfg, o.acc_init(gg, cfg), prec, strip_rows=strip,
+                           land_form=False)
+    for k in ("h", "hmax", "tmax"):
+        assert np.array_equal(getattr(go, k), getattr(ge, k)), k
+        assert np.array_equal(getattr(gg, k), getattr(ge, k)), k
+    assert [r.dt for r in log_o] == [r.dt for r in log_e] == [r.dt for r in log_g]
+    assert acc_o.cap_active == acc_e.cap_active == acc_g.cap_active
+    assert acc_o.V_rain == pytest.approx(acc_e.V_rain, rel=1e-12)
+    assert acc_o.V_sea == pytest.approx(acc_e.V_sea, rel=1e-12, abs=1e-9)
+    assert acc_o.V_bnd == pytest.approx(acc_e.V_bnd, rel=1e-12, abs=1e-9)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("K", [4, 8])
+def test_all_three_row_forms_match_oracle(prec, K):
+    """Half sea, half land on a wide grid: rows walked in the LAND form, the
+    no-LAND form (all-sea wavefronts) and the general form all occur, and the
+    run is the oracle's bit for bit."""
+    a, g, cfg = make_case(64, 300, prec=prec, K=K, T_sim=1500.0, land_frac=0.5, seed=5,
+                          h0=0.06)
+    o = Oracle(prec)
+    go, ge = g.copy(), g.copy()
+    acc_o, log_o = o.run(go, cfg, log_cap=100000)
+    forms = []
+    acc_e, log_e = emu_run(ge, cfg, o.acc_init(ge, cfg), prec, strip_rows=24, forms=forms)
+    assert min(forms) > 0, forms
+    for k in ("h", "hmax", "tmax"):
+        assert np.array_equal(getattr(go, k), getattr(ge, k)), k
+    assert [r.dt for r in log_o] == [r.dt for r in log_e]
+    assert acc_o.cap_active == acc_e.cap_active
+    assert acc_o.V_sea == pytest.approx(acc_e.V_sea, rel=1e-12, abs=1e-9)

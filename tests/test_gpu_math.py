@@ -19,18 +19,23 @@ TOP = 0x7F800000  # +inf bit pattern
 CHUNK = 1 << 26
 
 
+class McArgs(C.Structure):
+    _fields_ = [("g", C.c_float), ("d", C.c_float), ("rd", C.c_float), ("e", C.c_float),
+                ("f", C.c_float), ("dok", C.c_int), ("k", C.c_int)]
+
+
 def _lib():
     lib = C.CDLL(os.path.join(ROOT, "tests", "gpu", "libmathcheck.so"))
     lib.mc_eval.restype = C.c_int
-    lib.mc_eval.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float,
-                            C.c_int, C.c_void_p]
+    lib.mc_eval.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(McArgs), C.c_void_p]
     return lib
 
 
-def dev(lib, kind, first, count, g=9.81, d=90.0, rd=None, dok=1):
+def dev(lib, kind, first, count, g=9.81, d=90.0, rd=None, dok=1, e=0.0, f=0.0, k=0):
     out = np.empty(count, dtype=np.float32)
     rd = np.float32(1) / np.float32(d) if rd is None else rd
-    rc = lib.mc_eval(kind, first, count, g, d, float(rd), dok, out.ctypes.data)
+    a = McArgs(g, d, float(rd), e, f, dok, k)
+    rc = lib.mc_eval(kind, first, count, C.byref(a), out.ctypes.data)
     assert rc == 0, f"mc_eval failed: HIP error {rc}"
     return out
 
@@ -125,3 +130,105 @@ def test_rough_sqrt_is_within_the_filter_margin():
         worst = max(worst, float(rel.max()))
     print(f"rough sqrt max relative error {worst:.3e}")
     assert worst < 2.0 ** -21
+
+
+def _range(lo, hi):
+    lo = int(np.float32(lo).view(np.uint32))
+    hi = int(np.float32(hi).view(np.uint32))
+    for first in range(lo, hi + 1, CHUNK):
+        yield first, min(CHUNK, hi + 1 - first)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_sqrt_in_range_is_correctly_rounded_on_every_float_of_its_range():
+    lib = _lib()
+    for first, n in _range(2.0 ** -96, 2.0 ** 126):
+        got = dev(lib, 7, first, n)
+        x = bits(first, n).view(np.float32)
+        idx = same(got, np.sqrt(x))
+        assert idx.size == 0, f"sqrt mismatch at {hex(first + int(idx[0]))}"
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_rcp_in_range_is_correctly_rounded_on_every_float_of_its_range():
+    lib = _lib()
+    for first, n in _range(2.0 ** -100, 2.0 ** 100):
+        got = dev(lib, 8, first, n)
+        x = bits(first, n).view(np.float32)
+        idx = same(got, np.float32(1) / x)
+        assert idx.size == 0, f"rcp mismatch at {hex(first + int(idx[0]))}"
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("dconst", [90.0, float(np.float32(np.sqrt(2.0)) * np.float32(90.0))])
+def test_div_c_big_over_its_domain(dconst):
+    lib = _lib()
+    d = np.float32(dconst)
+    for first, n in _range(2.0 ** -80, 2.0 ** 80):
+        got = dev(lib, 9, first, n, d=float(d))
+        x = bits(first, n).view(np.float32)
+        idx = same(got, x / d)
+        assert idx.size == 0, f"x/{d} mismatch at {hex(first + int(idx[0]))}"
+
+
+@pytest.mark.gpu
+def test_div_c_keeps_signed_zeros():
+    lib = _lib()
+    for b in (0x00000000, 0x80000000):
+        got = dev(lib, 4, b, 1, d=8100.0)
+        assert got.view(np.uint32)[0] == b
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("k", [-300, -40, -3, -1, 0, 1, 2, 40])
+def test_crit_cap_equals_min_with_the_critical_speed(k):
+    """v stepped k ulps from the critical speed, for every depth in
+    [1e-3, 1e3]: the filtered cap returns exactly min(v, vcr)."""
+    lib = _lib()
+    for first, n in _range(1e-3, 1e3):
+        got = dev(lib, 10, first, n, k=k)
+        hf = bits(first, n).view(np.float32)
+        vcr = np.sqrt(np.float64(np.float32(9.81)) * hf.astype(np.float64)).astype(np.float32)
+        v = (vcr.view(np.int32) + np.int32(k)).view(np.float32)
+        idx = same(got, np.minimum(v, vcr))
+        assert idx.size == 0, f"k={k}: mismatch at {hex(first + int(idx[0]))}"
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("n,ax", [(0.02, 0.5), (0.15, 3e-4), (0.035, 0.0), (1e-13, 2.0),
+                                  (0.06, 40.0)])
+@pytest.mark.parametrize("k", [-1, -4, -4000, 1])
+def test_stage_d_bound_never_skips_a_faster_cell(n, ax, k):
+    """For every depth in [1e-3, 1e3] and thr one or more ulps below (k < 0) or
+    above the cell's exact speed, the bound never declares 'cannot exceed' for
+    a cell that does; for thr below the speed it must ask for the exact value."""
+    lib = _lib()
+    for first, cnt in _range(1e-3, 1e3):
+        got = dev(lib, 11, first, cnt, e=n, f=ax, k=k)
+        assert not np.any(got == 1.0), f"skipped a faster cell (n={n}, ax={ax}, k={k})"
+        if k < 0 and ax > 0:
+            assert np.all(got == 0.5)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("den", [480.0, 960.0, float(np.float32(8.0) * np.float32(52.371)),
+                                 float(np.float32(16.0) * np.float32(0.0137)),
+                                 float(np.float32(8.0) * np.float32(1.9999999))])
+def test_step_constant_division_is_correctly_rounded(den):
+    """div_step_cr (the level cap's |dL|*A / ((2K)*dt)) against IEEE division
+    for every dividend in [2^-80, 2^80], for step denominators including one
+    whose significand is all ones."""
+    lib = _lib()
+    d = np.float32(den)
+    rd = np.float32(1) / d
+    for first, n in _range(2.0 ** -80, 2.0 ** 80):
+        got = dev(lib, 12, first, n, d=float(d), rd=float(rd))
+        x = bits(first, n).view(np.float32)
+        idx = same(got, x / d)
+        assert idx.size == 0, f"x/{d} mismatch at {hex(first + int(idx[0]))}"
