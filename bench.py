@@ -85,14 +85,16 @@ class HipTimer:
 
 
 def load_traffic(workload: str):
-    """Per-launch HBM bytes measured with rocprofv3 --pmc for this workload,
-    as committed under profiles/ (None if not measured)."""
+    """Per-launch HBM bytes measured with rocprofv3 --pmc (FETCH_SIZE x2 +
+    WRITE_SIZE, scaled by what the counters read for copies of known size)
+    for this workload, as committed under profiles/ (None if not measured)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p, encoding="utf-8") as fp:
-            return json.load(fp).get(workload)
+            t = json.load(fp).get(workload)
     except (OSError, ValueError):
         return None
+    return None if t is None else float(t["bytes"])
 
 
 def cpu_baseline(seconds: float, prec: str):
@@ -200,7 +202,9 @@ def main() -> int:
     bps = bytes_per_step(n_land, n_all, prec)
     achieved = bps / (ms_per_step * 1e-3) / 1e9
     workload = f"fused-{prec}-K{args.neigh}-{args.size}"
-    traffic = load_traffic(workload) if kern == A.FCA_HIP_FUSED else None
+    # measured for the default kernel options only
+    traffic = (load_traffic(workload) if kern == A.FCA_HIP_FUSED and args.strip == 0
+               and not args.no_land_form else None)
     emass = abs(out.V + out.V_sea + out.V_bnd - out.V_rain - out.V0 - out.clamp_err) / \
         max(out.V_rain + out.V0, 1e-30)
     print(f"[bench] {ms_per_step:.4f} ms/step, {achieved:.0f} GB/s algorithmic, "
