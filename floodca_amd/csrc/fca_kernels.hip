@@ -18,6 +18,9 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <stdlib.h>
+#include <string.h>
+
 #include "fca_launch.h"
 #include "fca_walk.h"
 
@@ -230,8 +233,10 @@ struct FusedArgs {
 #define FCA_FUSED_ATTR
 #endif
 
+/* One workgroup of the fused step: wg of nwg workgroups walking a.ntiles
+ * tiles of one grid. */
 template <int K8>
-__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR k_fused(FusedArgs a)
+__device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
 {
     __shared__ StepVals<R> sh_sv;
     __shared__ int sh_hbuf;
@@ -242,11 +247,11 @@ __global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR k_fused(FusedArgs a)
      * lets the walk's row loop, row addresses and bounds live in scalar
      * registers */
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * (FCA_BLOCK / 64) + wave;
+    const int tile = wg * (FCA_BLOCK / 64) + wave;
 
     if (threadIdx.x == 0) {
         int hb;
-        sh_sv = ctl_step(a.io, a.ctl, a.p, blockIdx.x == 0, 1, &hb);
+        sh_sv = ctl_step(a.io, a.ctl, a.p, wg == 0, 1, &hb);
         sh_hbuf = hb;
     }
     __syncthreads();
@@ -262,7 +267,7 @@ __global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR k_fused(FusedArgs a)
         /* persistent wavefronts: wave w walks tiles w, w + W, w + 2W, ... in a
          * fixed order, so every workgroup's slot sums the same cells each step */
         WalkArgs<R> wa;
-        const int W = (int)gridDim.x * (FCA_BLOCK / 64);
+        const int W = nwg * (FCA_BLOCK / 64);
         wa.z = a.z, wa.n = a.n, wa.cr = a.cr, wa.mask = a.mask;
         wa.h_in = sh_hbuf ? a.h1 : a.h0;
         wa.h_out = sh_hbuf ? a.h0 : a.h1;
@@ -291,7 +296,7 @@ __global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR k_fused(FusedArgs a)
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        FcaAccSlot *sl = &a.slots[blockIdx.x];
+        FcaAccSlot *sl = &a.slots[wg];
         double vr = 0, cl = 0, vs = 0, vb = 0;
         long long cp = 0;
         R dm = (R)INFINITY;
@@ -305,6 +310,43 @@ __global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR k_fused(FusedArgs a)
             atomic_min_key(a.io.cand_out, cand_key(dm));
         }
     }
+}
+
+template <int K8>
+__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR k_fused(FusedArgs a)
+{
+    fused_body<K8>(a, blockIdx.x, gridDim.x);
+}
+
+/* ---- batched fused step ------------------------------------------------------
+ * Many grids (an ensemble of scenarios) advance one step each per launch: the
+ * launch's workgroups are listed as (scenario, workgroup within it), each
+ * scenario keeps its own controller, time step, candidate rotation and slots,
+ * and a workgroup runs exactly the code of the single-grid launch. */
+
+struct ScenDev {
+    FusedArgs a; /* io is set per launch from ctrl/cand */
+    FcaDevCtrl *ctrl;
+    Key *cand;
+    FcaHipStepRec *rec;
+    int rec_cap, nblocks;
+};
+
+template <int K8>
+__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR
+    k_fused_batch(const ScenDev *scen, const int2 *wgs, long long k)
+{
+    const int2 e = wgs[blockIdx.x];
+    const ScenDev *S = &scen[e.x];
+    FusedArgs a = S->a;
+    a.io.in = &S->ctrl[k & 1];
+    a.io.out = &S->ctrl[(k + 1) & 1];
+    a.io.cand_in = S->cand + (int)(k % 3);
+    a.io.cand_out = S->cand + (int)((k + 1) % 3);
+    a.io.cand_reset = S->cand + (int)((k + 2) % 3);
+    a.io.rec = S->rec;
+    a.io.rec_cap = S->rec_cap;
+    fused_body<K8>(a, e.y, S->nblocks);
 }
 
 /* ---- split step ----------------------------------------------------------- */
@@ -625,7 +667,9 @@ __global__ void __launch_bounds__(FCA_BLOCK)
         mx = (hm > mx) ? hm : mx;
         for (int j = 0; j < thr.n; j++) {
             if (hm >= thr.v[j]) {
-                counts[(size_t)j * N + i] += 1u;
+                if (counts) {
+                    counts[(size_t)j * N + i] += 1u;
+                }
                 f[j] += 1.0;
             }
         }
@@ -729,19 +773,26 @@ extern "C" int fca_k_cand_init(const FcaDev *d, int hbuf, int slot, hipStream_t 
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+static FusedArgs fused_args(const FcaDev *d)
+{
+    FusedArgs a;
+    memset(&a, 0, sizeof(a));
+    a.z = d->z, a.n = d->n, a.cr = d->cr, a.mask = d->mask;
+    a.h0 = d->h[0], a.h1 = d->h[1], a.hmax = d->hmax, a.tmax = d->tmax;
+    a.nr = d->nr, a.nc = d->nc, a.strip_rows = d->strip_rows;
+    a.ngroups = d->ngroups, a.ntiles = d->ntiles;
+    a.p = phys_of(d);
+    a.ctl = d->ctl;
+    a.slots = d->slots;
+    return a;
+}
+
 extern "C" int fca_k_step(const FcaDev *d, long long k, hipStream_t st)
 {
     const PhysK<R> p = phys_of(d);
     if (d->kernel == FCA_HIP_FUSED) {
-        FusedArgs a;
-        a.z = d->z, a.n = d->n, a.cr = d->cr, a.mask = d->mask;
-        a.h0 = d->h[0], a.h1 = d->h[1], a.hmax = d->hmax, a.tmax = d->tmax;
-        a.nr = d->nr, a.nc = d->nc, a.strip_rows = d->strip_rows;
-        a.ngroups = d->ngroups, a.ntiles = d->ntiles;
-        a.p = p;
-        a.ctl = d->ctl;
+        FusedArgs a = fused_args(d);
         a.io = ctl_io(d, k);
-        a.slots = d->slots;
         if (d->K == 8) {
             hipLaunchKernelGGL(k_fused<1>, dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
         } else {
@@ -772,6 +823,48 @@ extern "C" int fca_k_step(const FcaDev *d, long long k, hipStream_t st)
             hipLaunchKernelGGL(k_split_update<0>, g, b, 0, st, a);
             hipLaunchKernelGGL(k_split_cand<0>, g, b, 0, st, a);
         }
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" void *fca_k_batch_table(const FcaDev *const *devs, int n)
+{
+    ScenDev *h = (ScenDev *)calloc((size_t)(n > 0 ? n : 1), sizeof(ScenDev));
+    void *dv = NULL;
+    if (!h) {
+        return NULL;
+    }
+    for (int i = 0; i < n; i++) {
+        h[i].a = fused_args(devs[i]);
+        h[i].ctrl = devs[i]->ctrl;
+        h[i].cand = (Key *)devs[i]->cand;
+        h[i].rec = devs[i]->rec;
+        h[i].rec_cap = devs[i]->rec_cap;
+        h[i].nblocks = devs[i]->nblocks;
+    }
+    if (hipMalloc(&dv, (size_t)(n > 0 ? n : 1) * sizeof(ScenDev)) != hipSuccess ||
+        hipMemcpy(dv, h, (size_t)n * sizeof(ScenDev), hipMemcpyHostToDevice) != hipSuccess) {
+        if (dv) {
+            (void)hipFree(dv);
+        }
+        dv = NULL;
+    }
+    free(h);
+    return dv;
+}
+
+extern "C" int fca_k_step_batch(const void *tab, const void *wgs, int nwg, int K, long long k,
+                                hipStream_t st)
+{
+    if (nwg <= 0) {
+        return 0;
+    }
+    if (K == 8) {
+        hipLaunchKernelGGL(k_fused_batch<1>, dim3(nwg), dim3(FCA_BLOCK), 0, st,
+                           (const ScenDev *)tab, (const int2 *)wgs, k);
+    } else {
+        hipLaunchKernelGGL(k_fused_batch<0>, dim3(nwg), dim3(FCA_BLOCK), 0, st,
+                           (const ScenDev *)tab, (const int2 *)wgs, k);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -41,6 +41,14 @@ typedef struct FcaDev {
     int nred;
 } FcaDev;
 
+/* Device table of n grids for the batched fused step (NULL on failure); free
+ * with hipFree. */
+void *fca_k_batch_table(const FcaDev *const *devs, int n);
+/* One batched launch: nwg workgroups listed in wgs (device array of int2
+ * {grid index in the table, workgroup within that grid}); launch index k
+ * selects every grid's controller parity and candidate slots. */
+int fca_k_step_batch(const void *tab, const void *wgs, int nwg, int K, long long k,
+                     hipStream_t st);
 /* Fills the fused geometry of d from d->strip_rows. */
 void fca_k_geometry(FcaDev *d);
 /* Time-step candidates of the state in h[hbuf] into cand slot `slot`

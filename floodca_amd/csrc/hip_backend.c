@@ -66,6 +66,7 @@ struct FcaHipSim {
     double t_h2d, t_io;
     int n_ok;              /* every LAND roughness in the LAND form's range */
     int no_land_form;      /* FCA_HIP_NO_LAND_FORM was asked for */
+    int own_stream;        /* st was created for this session */
 };
 
 static void dev_free(FcaDev *d)
@@ -100,7 +101,7 @@ void fca_hip_sim_close(FcaHipSim *s)
     if (s->h_stage) {
         hipHostFree(s->h_stage);
     }
-    if (s->st) {
+    if (s->st && s->own_stream) {
         hipStreamDestroy(s->st);
     }
     free(s->cfg);
@@ -171,7 +172,7 @@ static void set_div_flags(FcaDev *d, int n_ok, int no_land_form)
 }
 
 static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
-                     const FcaHipOpts *o)
+                     const FcaHipOpts *o, hipStream_t shared)
 {
     FcaDev *d = &s->d;
     const size_t N = (size_t)g->nrows * (size_t)g->ncols, B = N * sizeof(real);
@@ -199,7 +200,13 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
     fca_k_geometry(d);
     s->N = N;
     s->batch = (o && o->graph_batch > 0) ? o->graph_batch : 32;
-    HIP_OR(hipStreamCreate(&s->st), -2);
+    if (shared) {
+        s->st = shared;
+        s->own_stream = 0;
+    } else {
+        HIP_OR(hipStreamCreate(&s->st), -2);
+        s->own_stream = 1;
+    }
     HIP_OR(hipMalloc((void **)&d->z, B), -2);
     HIP_OR(hipMalloc((void **)&d->n, B), -2);
     HIP_OR(hipMalloc((void **)&d->C, B), -2);
@@ -310,8 +317,8 @@ static int check_cfg(const FcaGrid *g, const FcaRunCfg *cfg, const FcaHipOpts *o
     return 0;
 }
 
-FcaHipSim *fca_hip_sim_open(const FcaGrid *g, const FcaRunCfg *cfg, const FcaAcc *acc,
-                            const FcaHipOpts *opts)
+static FcaHipSim *sim_create(const FcaGrid *g, const FcaRunCfg *cfg, const FcaAcc *acc,
+                             const FcaHipOpts *opts, hipStream_t shared)
 {
     FcaHipSim *s;
     g_err[0] = 0;
@@ -334,7 +341,7 @@ FcaHipSim *fca_hip_sim_open(const FcaGrid *g, const FcaRunCfg *cfg, const FcaAcc
     }
     memcpy(s->cfg, cfg, sizeof(FcaRunCfg));
     s->faces_per_step = count_faces(g, &cfg->phys);
-    if (sim_alloc(s, g, cfg, opts) != 0 || sim_upload_static(s, g) != 0 ||
+    if (sim_alloc(s, g, cfg, opts, shared) != 0 || sim_upload_static(s, g) != 0 ||
         sim_upload_state(s, g, acc) != 0) {
         char keep[512];
         memcpy(keep, g_err, sizeof(keep));
@@ -343,6 +350,12 @@ FcaHipSim *fca_hip_sim_open(const FcaGrid *g, const FcaRunCfg *cfg, const FcaAcc
         return NULL;
     }
     return s;
+}
+
+FcaHipSim *fca_hip_sim_open(const FcaGrid *g, const FcaRunCfg *cfg, const FcaAcc *acc,
+                            const FcaHipOpts *opts)
+{
+    return sim_create(g, cfg, acc, opts, NULL);
 }
 
 void *fca_hip_sim_stream(FcaHipSim *s)
@@ -888,4 +901,306 @@ void fca_hip_ens_close(FcaHipEns *e)
     }
     fca_hip_sim_close(e->s);
     free(e);
+}
+
+/* ---- batched session ------------------------------------------------------- */
+
+struct FcaHipBatch {
+    int n, K;
+    FcaHipSim **sims;
+    hipStream_t st;
+    FcaDevCtrl *ctrl_pool; /* device: 2 records per scenario, contiguous */
+    FcaDevCtrl *h_ctrl;    /* pinned copy of the pool */
+    void *tab;             /* device scenario table */
+    int *wgs_dev, *wgs_host;
+    int nwg, cap_wg;
+    char *active;
+    int *last_step;
+    long long k;
+    int batch;
+    double wall;
+};
+
+/* Workgroup list of the active scenarios: {scenario, workgroup} pairs. */
+static int batch_list(FcaHipBatch *b)
+{
+    int i, w, m = 0;
+    for (i = 0; i < b->n; i++) {
+        if (!b->active[i]) {
+            continue;
+        }
+        for (w = 0; w < b->sims[i]->d.nblocks; w++) {
+            b->wgs_host[2 * m] = i;
+            b->wgs_host[2 * m + 1] = w;
+            m++;
+        }
+    }
+    b->nwg = m;
+    if (m > 0) {
+        HIP_OR(hipMemcpyAsync(b->wgs_dev, b->wgs_host, (size_t)m * 2 * sizeof(int),
+                              hipMemcpyHostToDevice, b->st), -2);
+        HIP_OR(hipStreamSynchronize(b->st), -2);
+    }
+    return 0;
+}
+
+void fca_hip_batch_close(FcaHipBatch *b)
+{
+    int i;
+    if (!b) {
+        return;
+    }
+    if (b->st) {
+        hipStreamSynchronize(b->st);
+    }
+    for (i = 0; i < b->n; i++) {
+        if (b->sims && b->sims[i]) {
+            b->sims[i]->d.ctrl = NULL; /* a slice of the pool */
+            fca_hip_sim_close(b->sims[i]);
+        }
+    }
+    if (b->ctrl_pool) {
+        hipFree(b->ctrl_pool);
+    }
+    if (b->h_ctrl) {
+        hipHostFree(b->h_ctrl);
+    }
+    if (b->tab) {
+        hipFree(b->tab);
+    }
+    if (b->wgs_dev) {
+        hipFree(b->wgs_dev);
+    }
+    if (b->wgs_host) {
+        hipHostFree(b->wgs_host);
+    }
+    if (b->st) {
+        hipStreamDestroy(b->st);
+    }
+    free(b->sims);
+    free(b->active);
+    free(b->last_step);
+    free(b);
+}
+
+FcaHipBatch *fca_hip_batch_open(int n, const FcaGrid *const *g, const FcaRunCfg *const *cfg,
+                                const FcaAcc *acc, const FcaHipOpts *opts)
+{
+    FcaHipBatch *b;
+    FcaDev **devs = NULL;
+    int i;
+    g_err[0] = 0;
+    if (n <= 0 || !g || !cfg || !acc) {
+        fail(-1, "bad arguments");
+        return NULL;
+    }
+    if (opts && opts->kernel != FCA_HIP_FUSED) {
+        fail(-1, "the batched session runs the fused step only");
+        return NULL;
+    }
+    for (i = 0; i < n; i++) {
+        if (!g[i] || !cfg[i] || cfg[i]->phys.K != cfg[0]->phys.K) {
+            fail(-1, "scenario %d: missing, or K differs from scenario 0", i);
+            return NULL;
+        }
+        if (cfg[i]->n_force_days > 0) {
+            fail(-1, "scenario %d: multi-day forcing is not supported in a batch", i);
+            return NULL;
+        }
+    }
+    b = (FcaHipBatch *)calloc(1, sizeof(*b));
+    if (!b) {
+        fail(-2, "out of host memory");
+        return NULL;
+    }
+    b->n = n;
+    b->K = cfg[0]->phys.K;
+    b->batch = (opts && opts->graph_batch > 0) ? opts->graph_batch : 32;
+    b->sims = (FcaHipSim **)calloc((size_t)n, sizeof(FcaHipSim *));
+    b->active = (char *)calloc((size_t)n, 1);
+    b->last_step = (int *)calloc((size_t)n, sizeof(int));
+    devs = (FcaDev **)calloc((size_t)n, sizeof(FcaDev *));
+    if (!b->sims || !b->active || !b->last_step || !devs) {
+        fail(-2, "out of host memory");
+        goto fail_out;
+    }
+    if (hipStreamCreate(&b->st) != hipSuccess) {
+        fail(-2, "hipStreamCreate failed");
+        goto fail_out;
+    }
+    for (i = 0; i < n; i++) {
+        b->sims[i] = sim_create(g[i], cfg[i], &acc[i], opts, b->st);
+        if (!b->sims[i]) {
+            char keep[512];
+            snprintf(keep, sizeof(keep), "scenario %d: %s", i, g_err);
+            memcpy(g_err, keep, sizeof(keep));
+            goto fail_out;
+        }
+        b->cap_wg += b->sims[i]->d.nblocks;
+        b->active[i] = 1;
+        b->last_step[i] = acc[i].step;
+    }
+    /* one pool for every controller record, so a single copy reads them all */
+    if (hipMalloc((void **)&b->ctrl_pool, (size_t)n * 2 * sizeof(FcaDevCtrl)) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_ctrl, (size_t)n * 2 * sizeof(FcaDevCtrl),
+                      hipHostMallocDefault) != hipSuccess) {
+        fail(-2, "controller pool allocation failed");
+        goto fail_out;
+    }
+    for (i = 0; i < n; i++) {
+        FcaDev *d = &b->sims[i]->d;
+        if (hipMemcpy(&b->ctrl_pool[2 * i], d->ctrl, 2 * sizeof(FcaDevCtrl),
+                      hipMemcpyDeviceToDevice) != hipSuccess) {
+            fail(-2, "controller pool copy failed");
+            goto fail_out;
+        }
+        hipFree(d->ctrl);
+        d->ctrl = &b->ctrl_pool[2 * i];
+        devs[i] = d;
+    }
+    b->tab = fca_k_batch_table((const FcaDev *const *)devs, n);
+    if (!b->tab) {
+        fail(-2, "scenario table upload failed");
+        goto fail_out;
+    }
+    if (hipMalloc((void **)&b->wgs_dev, (size_t)b->cap_wg * 2 * sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void **)&b->wgs_host, (size_t)b->cap_wg * 2 * sizeof(int),
+                      hipHostMallocDefault) != hipSuccess) {
+        fail(-2, "workgroup list allocation failed");
+        goto fail_out;
+    }
+    if (batch_list(b) != 0) {
+        goto fail_out;
+    }
+    free(devs);
+    return b;
+fail_out:
+    free(devs);
+    fca_hip_batch_close(b);
+    return NULL;
+}
+
+int fca_hip_batch_nscen(const FcaHipBatch *b)
+{
+    return b ? b->n : 0;
+}
+
+/* Runs every scenario to its end (T_sim, or cfg->steps): batches of launches
+ * advance all active scenarios one step each per launch; between batches one
+ * copy reads all controller records and finished scenarios leave the
+ * workgroup list. */
+int fca_hip_batch_run(FcaHipBatch *b)
+{
+    int i, j, nact;
+    double t0;
+    if (!b) {
+        return fail(-1, "NULL batch");
+    }
+    t0 = now_s();
+    for (;;) {
+        int changed = 0;
+        nact = 0;
+        for (i = 0; i < b->n; i++) {
+            nact += b->active[i];
+        }
+        if (nact == 0) {
+            break;
+        }
+        for (j = 0; j < b->batch; j++) {
+            if (fca_k_step_batch(b->tab, b->wgs_dev, b->nwg, b->K, b->k, b->st) != 0) {
+                return fail(-2, "batched step launch failed: %s",
+                            hipGetErrorString(hipGetLastError()));
+            }
+            b->k++;
+        }
+        HIP_OR(hipMemcpyAsync(b->h_ctrl, b->ctrl_pool, (size_t)b->n * 2 * sizeof(FcaDevCtrl),
+                              hipMemcpyDeviceToHost, b->st), -2);
+        HIP_OR(hipStreamSynchronize(b->st), -2);
+        for (i = 0; i < b->n; i++) {
+            const FcaRunCfg *cfg = b->sims[i]->cfg;
+            const FcaDevCtrl *c = &b->h_ctrl[2 * i + (int)(b->k & 1)];
+            int done;
+            if (!b->active[i]) {
+                continue;
+            }
+            if (c->state == FCA_CTL_PAUSE) {
+                return fail(-2, "scenario %d paused for forcing", i);
+            }
+            /* a fixed-step scenario that made no progress in a whole batch
+             * cannot advance (the reference would spin) */
+            done = c->state == FCA_CTL_DONE ||
+                   (cfg->steps > 0 && (c->step >= cfg->steps || c->step == b->last_step[i])) ||
+                   (cfg->steps <= 0 && c->t >= (double)cfg->T_sim - 1e-15);
+            b->last_step[i] = c->step;
+            if (done) {
+                b->active[i] = 0;
+                changed = 1;
+            }
+        }
+        if (changed && batch_list(b) != 0) {
+            return -2;
+        }
+    }
+    for (i = 0; i < b->n; i++) {
+        b->sims[i]->k = b->k;
+    }
+    b->wall += now_s() - t0;
+    return 0;
+}
+
+int fca_hip_batch_read(FcaHipBatch *b, int i, real *h, real *hmax, real *tmax, FcaAcc *acc)
+{
+    if (!b || i < 0 || i >= b->n) {
+        return fail(-1, "bad arguments");
+    }
+    b->sims[i]->k = b->k;
+    return fca_hip_sim_read(b->sims[i], h, hmax, tmax, acc);
+}
+
+int fca_hip_batch_steplog(FcaHipBatch *b, int i, FcaHipStepRec *out, int cap)
+{
+    if (!b || i < 0 || i >= b->n) {
+        return fail(-1, "bad arguments");
+    }
+    b->sims[i]->k = b->k;
+    return fca_hip_sim_steplog(b->sims[i], out, cap);
+}
+
+/* LAND cells of scenario i with hmax >= thr[j] and the largest LAND hmax
+ * (ens_common.c:166-188), reduced on the device. */
+int fca_hip_batch_flooded(FcaHipBatch *b, int i, int nthr, const double *thr, long *flooded,
+                          double *max_depth)
+{
+    FcaHipSim *s;
+    int nb, k, j;
+    if (!b || i < 0 || i >= b->n || nthr < 0 || nthr > 4 || (nthr > 0 && (!thr || !flooded)) ||
+        !max_depth) {
+        return fail(-1, "bad arguments");
+    }
+    s = b->sims[i];
+    nb = fca_k_ens_count(&s->d, NULL, nthr, thr, b->st);
+    if (nb < 0) {
+        return fail(-2, "count launch failed");
+    }
+    HIP_OR(hipMemcpyAsync(s->h_red, s->d.red, (size_t)nb * 5 * sizeof(double),
+                          hipMemcpyDeviceToHost, b->st), -2);
+    HIP_OR(hipStreamSynchronize(b->st), -2);
+    for (j = 0; j < nthr; j++) {
+        double f = 0.0;
+        for (k = 0; k < nb; k++) {
+            f += s->h_red[(size_t)k * 5 + j];
+        }
+        flooded[j] = (long)f;
+    }
+    *max_depth = 0.0;
+    for (k = 0; k < nb; k++) {
+        const double m = s->h_red[(size_t)k * 5 + 4];
+        *max_depth = m > *max_depth ? m : *max_depth;
+    }
+    return 0;
+}
+
+double fca_hip_batch_wall(const FcaHipBatch *b)
+{
+    return b ? b->wall : 0.0;
 }

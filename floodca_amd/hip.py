@@ -54,6 +54,18 @@ class Backend:
             "fca_hip_ens_counts": (C.c_int, [C.c_void_p, C.c_void_p]),
             "fca_hip_ens_counts_u32": (C.c_int, [C.c_void_p, C.c_void_p]),
             "fca_hip_ens_close": (None, [C.c_void_p]),
+            "fca_hip_batch_open": (C.c_void_p, [C.c_int, C.POINTER(G), C.POINTER(Cf), Ac,
+                                                C.POINTER(a.FcaHipOpts)]),
+            "fca_hip_batch_nscen": (C.c_int, [C.c_void_p]),
+            "fca_hip_batch_run": (C.c_int, [C.c_void_p]),
+            "fca_hip_batch_read": (C.c_int, [C.c_void_p, C.c_int, P, P, P, Ac]),
+            "fca_hip_batch_steplog": (C.c_int, [C.c_void_p, C.c_int,
+                                                C.POINTER(a.FcaHipStepRec), C.c_int]),
+            "fca_hip_batch_flooded": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
+                                                C.POINTER(C.c_double), C.POINTER(C.c_long),
+                                                C.POINTER(C.c_double)]),
+            "fca_hip_batch_wall": (C.c_double, [C.c_void_p]),
+            "fca_hip_batch_close": (None, [C.c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -201,6 +213,78 @@ class Ensemble:
         if self.h:
             self.b.lib.fca_hip_ens_close(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """fca_hip_batch_*: scenarios resident at once, advanced together."""
+
+    def __init__(self, grids, cfgs, accs, prec="f32", **kw):
+        self.b = backend(prec)
+        self.a = a = self.b.a
+        n = len(grids)
+        if not (n == len(cfgs) == len(accs)) or n == 0:
+            raise ValueError("need one cfg and one acc per grid")
+        self.grids = list(grids)
+        self.opts = opts(prec, **kw)
+        self._g = [g.fca_grid(a) for g in grids]
+        self._gp = (C.POINTER(a.FcaGrid) * n)(*[C.pointer(x) for x in self._g])
+        self._c = list(cfgs)
+        self._cp = (C.POINTER(a.FcaRunCfg) * n)(*[C.pointer(x) for x in self._c])
+        self._acc = (a.FcaAcc * n)(*accs)
+        self.h = self.b.lib.fca_hip_batch_open(n, self._gp, self._cp, self._acc,
+                                               C.byref(self.opts))
+        if not self.h:
+            raise HipError(f"fca_hip_batch_open: {self.b.error()}")
+        self.n = n
+
+    def run(self):
+        self.b.check(self.b.lib.fca_hip_batch_run(self.h), "batch run")
+
+    @property
+    def wall(self) -> float:
+        return self.b.lib.fca_hip_batch_wall(self.h)
+
+    def read(self, i, grid=None):
+        """Copies scenario i's h, hmax, tmax into grid (default: its own) and
+        returns its FcaAcc."""
+        grid = grid or self.grids[i]
+        acc = self.a.FcaAcc()
+        P = C.POINTER(self.a.real)
+        self.b.check(self.b.lib.fca_hip_batch_read(self.h, i, grid.h.ctypes.data_as(P),
+                                                   grid.hmax.ctypes.data_as(P),
+                                                   grid.tmax.ctypes.data_as(P), C.byref(acc)),
+                     "batch read")
+        return acc
+
+    def steplog(self, i, cap):
+        buf = (self.a.FcaHipStepRec * max(cap, 1))()
+        n = self.b.check(self.b.lib.fca_hip_batch_steplog(self.h, i, buf, cap), "steplog")
+        return list(buf)[:n]
+
+    def flooded(self, i, thresholds=(0.10, 0.25, 0.50)):
+        thr = (C.c_double * len(thresholds))(*thresholds)
+        fl = (C.c_long * len(thresholds))()
+        md = C.c_double(0)
+        self.b.check(self.b.lib.fca_hip_batch_flooded(self.h, i, len(thresholds), thr, fl,
+                                                      C.byref(md)), "flooded")
+        return list(fl), md.value
+
+    def close(self):
+        if self.h:
+            self.b.lib.fca_hip_batch_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

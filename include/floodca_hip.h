@@ -289,6 +289,31 @@ int fca_hip_sim_run(FcaHipSim *s);
 void *fca_hip_sim_stream(FcaHipSim *s);
 void fca_hip_sim_close(FcaHipSim *s);
 
+/* Batched scenarios: n grids (sizes may differ, K may not) resident at once,
+ * each with its own configuration, time step and accumulators; one launch
+ * advances every unfinished scenario by one step, so small grids share the
+ * GPU instead of each filling a fraction of it. Results are those of
+ * running each scenario alone, bit for bit. acc is an array of n (V0, t,
+ * step, force_day, dt_base as for fca_hip_sim_open); multi-day forcing is not
+ * supported. No counterpart in the reference (its ensemble runs scenarios one
+ * after another, kernels.cu:930-997). */
+typedef struct FcaHipBatch FcaHipBatch;
+FcaHipBatch *fca_hip_batch_open(int n, const FcaGrid *const *g, const FcaRunCfg *const *cfg,
+                                const FcaAcc *acc, const FcaHipOpts *opts);
+int fca_hip_batch_nscen(const FcaHipBatch *b);
+/* Runs every scenario to its end (T_sim, or cfg->steps). Returns 0 or < 0. */
+int fca_hip_batch_run(FcaHipBatch *b);
+/* As fca_hip_sim_read / fca_hip_sim_steplog for scenario i. */
+int fca_hip_batch_read(FcaHipBatch *b, int i, real *h, real *hmax, real *tmax, FcaAcc *acc);
+int fca_hip_batch_steplog(FcaHipBatch *b, int i, FcaHipStepRec *out, int cap);
+/* Scenario i's LAND cells with hmax >= thr[j] (nthr <= 4) and its largest
+ * LAND hmax, as the ensemble reports them (ens_common.c:166-188). */
+int fca_hip_batch_flooded(FcaHipBatch *b, int i, int nthr, const double *thr, long *flooded,
+                          double *max_depth);
+/* Seconds spent in fca_hip_batch_run so far. */
+double fca_hip_batch_wall(const FcaHipBatch *b);
+void fca_hip_batch_close(FcaHipBatch *b);
+
 #ifdef __cplusplus
 }
 #endif
