@@ -141,6 +141,46 @@ def cpu_baseline(seconds: float, prec: str):
                       f"50 mm/h rain, FP32, serial, wall {wall:.1f} s"}
 
 
+def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261204):
+    """Scenario ensemble on crops of base (floodca_amd.ensemble): all n_scen
+    scenarios in one batched session, and a seeded sample of n_alone of them
+    run one session each (the per-scenario baseline) and batched again, whose
+    results must be identical. Scenarios/h counts upload, run and per-scenario
+    reductions."""
+    from floodca_amd import ensemble as E
+    t0 = time.time()
+    scen = E.draw(n_scen, seed, base.shape)
+    cases = [E.build(base, s, prec) for s in scen]
+    cells = sum(g.z.size for g, _ in cases)
+    t_build = time.time() - t0
+    strip, tune = E.autotune_strip(cases, prec)
+    wall = []
+    t_b, r_b = E.run_batched(cases, prec, strip_rows=strip, run_wall=wall)
+    rng = np.random.default_rng(seed + 1)
+    idx = sorted(rng.choice(n_scen, size=min(n_alone, n_scen), replace=False).tolist())
+    sub = [cases[i] for i in idx]
+    t_a, r_a = E.run_alone(sub, prec)
+    t_s, r_s = E.run_batched(sub, prec, strip_rows=strip)
+    same = r_a == r_s == [r_b[i] for i in idx]
+    steps = [r[2] for r in r_b]
+    print(f"[bench] ensemble: {n_scen} scenarios ({cells / 1e6:.0f} M cells) built in "
+          f"{t_build:.1f}s, strip {strip} ({ {k: round(v, 3) for k, v in tune.items()} }), "
+          f"batched {t_b:.2f}s (run {wall[0]:.2f}s); sample of {len(idx)}: alone {t_a:.2f}s, "
+          f"batched {t_s:.2f}s, identical={same}", file=sys.stderr, flush=True)
+    return {
+        "scenarios_per_h": n_scen / t_b * 3600.0,
+        "sample_alone_scenarios_per_h": len(idx) / t_a * 3600.0,
+        "sample_batched_scenarios_per_h": len(idx) / t_s * 3600.0,
+        "speedup_vs_alone": t_a / t_s,
+        "results_identical": bool(same),
+        "scenarios": n_scen, "sample": len(idx), "cells": cells,
+        "steps_min_max": [min(steps), max(steps)], "strip_rows": strip,
+        "workload": "crops of the 16384^2 DEM, side U[256, 2048]; triangular storm "
+                    "D U[2, 12] h, 50 mm/h x lognormal(0, 0.5); Manning x U[0.8, 1.2] per "
+                    "class; T_sim = D; FP32, K=4",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +193,10 @@ def main() -> int:
     ap.add_argument("--neigh", type=int, default=4, choices=(4, 8))
     ap.add_argument("--no-land-form", action="store_true",
                     help="fused walk in its general form only (same results)")
+    ap.add_argument("--ens-scenarios", type=int, default=96,
+                    help="scenarios of the ensemble measured after the main line (0 = skip)")
+    ap.add_argument("--ens-alone", type=int, default=24,
+                    help="of which this many are also run one session each")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -240,6 +284,8 @@ def main() -> int:
             "bytes_rule": "per LAND cell z,n,cr,h,hmax + mask read, h written; 1 B mask per other cell",
         },
     }
+    if args.ens_scenarios > 0 and prec == "f32" and args.neigh == 4:
+        line["ensemble"] = ensemble_bench(g, prec, args.ens_scenarios, args.ens_alone)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds, prec)
     print(json.dumps(line), flush=True)

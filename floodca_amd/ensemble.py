@@ -1,0 +1,172 @@
+"""Scenario ensembles on one GPU: the batched session against one run per
+scenario.
+
+The workload follows the reference's master-worker ensemble
+(src/hybrid/main_ens_mpi_cuda.c:71-120, ens_common.c:166-258): every scenario
+is a full run from a dry start with its own forcing, and reports its flooded
+LAND cells at the thresholds {0.10, 0.25, 0.50} m and its largest depth. Here
+scenarios also differ in grid: square crops of one large synthetic DEM with
+uneven sides, so a batch mixes grid sizes and time steps.
+
+Scenario draw (seeded):
+  * crop: side uniform in [side_lo, side_hi], position uniform in the big grid;
+  * rain: a triangular hyetograph (rain.c:13-28) of duration D uniform in
+    [dur_lo, dur_hi] hours with its peak at mid-storm (phi = 0.5), mean
+    intensity 50 mm/h times a lognormal factor (median 1, sigma 0.5), so the
+    storm depth Rday = 0.05 m/h * factor * D (C = 1); the run lasts the storm,
+    T_sim = D;
+  * roughness: each of the 8 land-cover classes scaled by U[0.8, 1.2].
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi as A
+from .grid import Grid
+from .hip import Batch
+from .synth import MANNING_CLASSES
+
+THRESHOLDS = (0.10, 0.25, 0.50)
+
+
+@dataclass
+class Scenario:
+    idx: int
+    r0: int
+    c0: int
+    side: int
+    factor: float
+    dur_h: float
+    nscale: np.ndarray
+
+
+def draw(n: int, seed: int, big: tuple, side=(256, 2048), dur=(2.0, 12.0)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        s = int(rng.integers(side[0], side[1] + 1))
+        s = min(s, big[0], big[1])
+        out.append(Scenario(idx=i, r0=int(rng.integers(0, big[0] - s + 1)),
+                            c0=int(rng.integers(0, big[1] - s + 1)), side=s,
+                            factor=float(rng.lognormal(0.0, 0.5)),
+                            dur_h=float(rng.uniform(dur[0], dur[1])),
+                            nscale=rng.uniform(0.8, 1.2, MANNING_CLASSES.size)))
+    return out
+
+
+def build(base: Grid, sc: Scenario, prec: str = "f32"):
+    """The scenario's grid (a crop of base with scaled roughness) and config."""
+    a = A.abi(prec)
+    dt = a.dtype
+    sl = (slice(sc.r0, sc.r0 + sc.side), slice(sc.c0, sc.c0 + sc.side))
+    mask = np.ascontiguousarray(base.mask[sl])
+    n0 = base.n[sl]
+    cls = np.searchsorted(MANNING_CLASSES.astype(n0.dtype), n0).clip(0, MANNING_CLASSES.size - 1)
+    n = (MANNING_CLASSES[cls] * sc.nscale[cls]).astype(dt)
+    shape = mask.shape
+    depth = 0.05 * sc.factor * sc.dur_h
+    g = Grid(z=np.ascontiguousarray(base.z[sl], dtype=dt), mask=mask, n=n,
+             C=np.ones(shape, dtype=dt), Rday=np.full(shape, depth, dtype=dt),
+             h=np.zeros(shape, dtype=dt), dx=base.dx)
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, g.dx)
+    cfg.hyeto = A.FCA_HYETO_TRIANGLE
+    cfg.D_h, cfg.phi = sc.dur_h, 0.5
+    cfg.T_sim = sc.dur_h * 3600.0
+    cfg.steps = 0
+    return g, cfg
+
+
+def acc_start(prec: str, g: Grid, cfg):
+    """Accumulators of a run from g's state (fca_acc_init, model.c:102-109):
+    V0 = V = sum of LAND h*A in double, force_day = -1, the rest zero."""
+    a = A.abi(prec)
+    acc = a.FcaAcc()
+    land = g.mask == A.FCA_LAND
+    acc.V0 = acc.V = float(np.sum(g.h[land].astype(np.float64) * float(cfg.phys.A)))
+    acc.force_day = -1
+    return acc
+
+
+def run_batched(cases, prec="f32", strip_rows=0, run_wall=None):
+    """One batch over all cases [(grid, cfg)]: returns (seconds, per-scenario
+    (flooded, max depth, steps)). The time covers upload, the run and the
+    per-scenario reductions; run_wall, if a list, receives the seconds of the
+    run alone."""
+    t0 = time.perf_counter()
+    accs = [acc_start(prec, g, c) for g, c in cases]
+    res = []
+    with Batch([g for g, _ in cases], [c for _, c in cases], accs, prec=prec,
+               strip_rows=strip_rows) as b:
+        b.run()
+        for i in range(len(cases)):
+            fl, md = b.flooded(i, THRESHOLDS)
+            acc = b.read(i, grid=_Sink(cases[i][0]))
+            res.append((fl, md, acc.step))
+        if run_wall is not None:
+            run_wall.append(b.wall)
+    return time.perf_counter() - t0, res
+
+
+def strip_alone(nr: int, nc: int) -> int:
+    """Strip height for a grid run on its own: short enough that the grid
+    yields ~4096 wavefront tiles (the GPU's resident wavefronts at this
+    kernel's occupancy), at least 8 rows and at most 96."""
+    groups = (nc + 57) // 58
+    return int(max(8, min(96, nr * groups // 4096)))
+
+
+def run_alone(cases, prec="f32", strip_rows=None):
+    """Each case in a session of its own, one after another (the reference's
+    per-scenario ensemble): returns (seconds, per-scenario results). With
+    strip_rows None each grid takes strip_alone's height."""
+    t0 = time.perf_counter()
+    res = []
+    for g, c in cases:
+        s = strip_alone(*g.shape) if strip_rows is None else strip_rows
+        with Batch([g], [c], [acc_start(prec, g, c)], prec=prec, strip_rows=s) as b:
+            b.run()
+            fl, md = b.flooded(0, THRESHOLDS)
+            acc = b.read(0, grid=_Sink(g))
+            res.append((fl, md, acc.step))
+    return time.perf_counter() - t0, res
+
+
+class _Sink:
+    """Read target that keeps h/hmax/tmax in fresh arrays, so reading a
+    scenario back does not overwrite its input grid."""
+
+    def __init__(self, g: Grid):
+        self.h = np.empty_like(g.h)
+        self.hmax = np.empty_like(g.hmax)
+        self.tmax = np.empty_like(g.tmax)
+
+
+def _with_tsim(cfg, t_sim):
+    c = type(cfg)()
+    ctypes.memmove(ctypes.addressof(c), ctypes.addressof(cfg), ctypes.sizeof(cfg))
+    c.T_sim = t_sim
+    return c
+
+
+def autotune_strip(cases, prec="f32", candidates=(32, 48, 64, 96, 128), sample=24, seed=0):
+    """Strip height for a batch: the candidate with the shortest batched run
+    of a seeded sample of the cases, each cut to its first 20 minutes of
+    simulated time (the run only, not the upload). Results do not depend on
+    the strip height (the walk's arithmetic is per cell), only the time does.
+    Returns (best, {candidate: seconds})."""
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(len(cases), size=min(sample, len(cases)), replace=False)
+    sub = [(cases[i][0], _with_tsim(cases[i][1], min(float(cases[i][1].T_sim), 1200.0)))
+           for i in idx]
+    timings = {}
+    for s in candidates:
+        wall = []
+        run_batched(sub, prec, strip_rows=s, run_wall=wall)
+        timings[s] = wall[0]
+    best = min(timings, key=timings.get)
+    return best, timings
