@@ -154,8 +154,8 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
     cells = sum(g.z.size for g, _ in cases)
     t_build = time.time() - t0
     strip, tune = E.autotune_strip(cases, prec)
-    wall = []
-    t_b, r_b = E.run_batched(cases, prec, strip_rows=strip, run_wall=wall)
+    wall, ph = [], {}
+    t_b, r_b = E.run_batched(cases, prec, strip_rows=strip, run_wall=wall, phases=ph)
     rng = np.random.default_rng(seed + 1)
     idx = sorted(rng.choice(n_scen, size=min(n_alone, n_scen), replace=False).tolist())
     sub = [cases[i] for i in idx]
@@ -165,7 +165,8 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
     steps = [r[2] for r in r_b]
     print(f"[bench] ensemble: {n_scen} scenarios ({cells / 1e6:.0f} M cells) built in "
           f"{t_build:.1f}s, strip {strip} ({ {k: round(v, 3) for k, v in tune.items()} }), "
-          f"batched {t_b:.2f}s (run {wall[0]:.2f}s); sample of {len(idx)}: alone {t_a:.2f}s, "
+          f"batched {t_b:.2f}s ({ {k: round(v, 2) for k, v in ph.items()} }); "
+          f"sample of {len(idx)}: alone {t_a:.2f}s, "
           f"batched {t_s:.2f}s, identical={same}", file=sys.stderr, flush=True)
     return {
         "scenarios_per_h": n_scen / t_b * 3600.0,
@@ -173,6 +174,7 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
         "sample_batched_scenarios_per_h": len(idx) / t_s * 3600.0,
         "speedup_vs_alone": t_a / t_s,
         "results_identical": bool(same),
+        "batched_phases_s": {k: round(v, 3) for k, v in ph.items()},
         "scenarios": n_scen, "sample": len(idx), "cells": cells,
         "steps_min_max": [min(steps), max(steps)], "strip_rows": strip,
         "workload": "crops of the 16384^2 DEM, side U[256, 2048]; triangular storm "

@@ -59,15 +59,54 @@ struct FcaHipSim {
     FcaAcc acc0;           /* accumulators at upload time */
     long faces_per_step;   /* faces whose cap flag the reference counts per step */
     int batch;             /* launches between controller read-backs */
-    FcaDevCtrl *h_ctrl;    /* pinned copy of a controller record */
-    double *h_red;         /* pinned copy of reduction partials */
+    FcaDevCtrl *h_ctrl;    /* host copy of a controller record */
+    double *h_red;         /* host copy of reduction partials */
     real *h_stage;         /* pinned staging for forcing reloads */
     FcaRunCfg *cfg;        /* copy of the run configuration */
     double t_h2d, t_io;
     int n_ok;              /* every LAND roughness in the LAND form's range */
     int no_land_form;      /* FCA_HIP_NO_LAND_FORM was asked for */
     int own_stream;        /* st was created for this session */
+    hipGraphExec_t gexec;  /* glen step launches, captured once */
+    int glen;
 };
+
+/* Launch indices enter a launch only through k & 1 and k % 3, so a capture of
+ * launches 0..n-1 with n a multiple of 6 replays as launches k..k+n-1 for
+ * any k that is a multiple of 6. */
+#define FCA_GRAPH_PERIOD 6
+
+static int capture_steps(hipStream_t st, hipGraphExec_t *exec, int n,
+                         int (*launch)(void *ctx, long long k), void *ctx)
+{
+    hipGraph_t gr = NULL;
+    int j, rc = 0;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        return fail(-2, "hipStreamBeginCapture failed");
+    }
+    for (j = 0; j < n && rc == 0; j++) {
+        rc = launch(ctx, j);
+    }
+    if (hipStreamEndCapture(st, &gr) != hipSuccess || rc != 0 || !gr) {
+        if (gr) {
+            hipGraphDestroy(gr);
+        }
+        return fail(-2, "step capture failed");
+    }
+    if (hipGraphInstantiate(exec, gr, NULL, NULL, 0) != hipSuccess) {
+        hipGraphDestroy(gr);
+        *exec = NULL;
+        return fail(-2, "hipGraphInstantiate failed");
+    }
+    hipGraphDestroy(gr);
+    return 0;
+}
+
+static int launch_sim_step(void *ctx, long long k)
+{
+    FcaHipSim *s = (FcaHipSim *)ctx;
+    return fca_k_step(&s->d, k, s->st);
+}
 
 static void dev_free(FcaDev *d)
 {
@@ -92,14 +131,13 @@ void fca_hip_sim_close(FcaHipSim *s)
         hipStreamSynchronize(s->st);
     }
     dev_free(&s->d);
-    if (s->h_ctrl) {
-        hipHostFree(s->h_ctrl);
-    }
-    if (s->h_red) {
-        hipHostFree(s->h_red);
-    }
+    free(s->h_ctrl);
+    free(s->h_red);
     if (s->h_stage) {
         hipHostFree(s->h_stage);
+    }
+    if (s->gexec) {
+        hipGraphExecDestroy(s->gexec);
     }
     if (s->st && s->own_stream) {
         hipStreamDestroy(s->st);
@@ -156,15 +194,33 @@ static int div_const_exact(double dd)
 #endif
 }
 
+/* div_const_exact with the answers remembered: a check walks 2^23 mantissas,
+ * and a batch opens hundreds of sessions over the same few divisors. */
+static int div_const_exact_cached(double dd)
+{
+    static double keys[16];
+    static int vals[16], used;
+    int i;
+    for (i = 0; i < used; i++) {
+        if (keys[i] == dd) {
+            return vals[i];
+        }
+    }
+    i = used < 16 ? used++ : 15;
+    keys[i] = dd;
+    vals[i] = div_const_exact(dd);
+    return vals[i];
+}
+
 /* The divisors of the step: dx, (real)sqrt(2.0)*dx (physics.h:45-48), A;
  * and whether the fused walk may take its LAND form (n_ok: every LAND
  * roughness in range, from the static upload). */
 static void set_div_flags(FcaDev *d, int n_ok, int no_land_form)
 {
     const real dx = d->phys.dx, ddiag = (real)sqrt(2.0) * dx;
-    d->fdiv[0] = div_const_exact((double)dx);
-    d->fdiv[1] = d->phys.K == 8 ? div_const_exact((double)ddiag) : 0;
-    d->fdiv[2] = div_const_exact((double)d->phys.A);
+    d->fdiv[0] = div_const_exact_cached((double)dx);
+    d->fdiv[1] = d->phys.K == 8 ? div_const_exact_cached((double)ddiag) : 0;
+    d->fdiv[2] = div_const_exact_cached((double)d->phys.A);
     d->land_form = !no_land_form && n_ok && fca_land_phys_ok(&d->phys);
 #ifndef FCA_DOUBLE
     d->land_form = d->land_form && d->fdiv[0] && d->fdiv[2] && (d->phys.K != 8 || d->fdiv[1]);
@@ -199,7 +255,9 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
     d->max_blocks = (o && o->max_blocks > 0) ? o->max_blocks : 0;
     fca_k_geometry(d);
     s->N = N;
-    s->batch = (o && o->graph_batch > 0) ? o->graph_batch : 32;
+    s->batch = (o && o->graph_batch > 0) ? o->graph_batch : 30;
+    /* replays of glen launches; graph_batch 1 launches every step itself */
+    s->glen = (s->batch / FCA_GRAPH_PERIOD) * FCA_GRAPH_PERIOD;
     if (shared) {
         s->st = shared;
         s->own_stream = 0;
@@ -237,9 +295,13 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
         HIP_OR(hipMalloc((void **)&d->sv, 256), -2);
         HIP_OR(hipMemset(d->qo, 0, 4 * B), -2);
     }
-    HIP_OR(hipHostMalloc((void **)&s->h_ctrl, 2 * sizeof(FcaDevCtrl), hipHostMallocDefault), -2);
-    HIP_OR(hipHostMalloc((void **)&s->h_red, (size_t)d->nred * sizeof(double),
-                         hipHostMallocDefault), -2);
+    /* small read-backs go to ordinary memory: pinning costs more than it
+     * saves at these sizes, and a batch opens hundreds of sessions */
+    s->h_ctrl = (FcaDevCtrl *)malloc(2 * sizeof(FcaDevCtrl));
+    s->h_red = (double *)malloc((size_t)d->nred * sizeof(double));
+    if (!s->h_ctrl || !s->h_red) {
+        return fail(-2, "out of host memory");
+    }
     return 0;
 }
 
@@ -365,15 +427,28 @@ void *fca_hip_sim_stream(FcaHipSim *s)
 
 int fca_hip_sim_advance(FcaHipSim *s, int nsteps)
 {
-    int i;
+    int left;
     if (!s || nsteps < 0) {
         return fail(-1, "bad arguments");
     }
-    for (i = 0; i < nsteps; i++) {
+    left = nsteps;
+    while (left > 0) {
+        if (s->glen > 0 && s->k % FCA_GRAPH_PERIOD == 0 && left >= s->glen) {
+            if (!s->gexec && capture_steps(s->st, &s->gexec, s->glen, launch_sim_step, s) != 0) {
+                return -2;
+            }
+            if (hipGraphLaunch(s->gexec, s->st) != hipSuccess) {
+                return fail(-2, "graph launch failed");
+            }
+            s->k += s->glen;
+            left -= s->glen;
+            continue;
+        }
         if (fca_k_step(&s->d, s->k, s->st) != 0) {
             return fail(-2, "step launch failed: %s", hipGetErrorString(hipGetLastError()));
         }
         s->k++;
+        left--;
     }
     return nsteps;
 }
@@ -802,6 +877,10 @@ int fca_hip_ens_run(FcaHipEns *e, FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc,
     memcpy(s->cfg, cfg, sizeof(FcaRunCfg));
     s->d.phys = cfg->phys;
     set_div_flags(&s->d, s->n_ok, s->no_land_form);
+    if (s->gexec) { /* captured with the previous scenario's parameters */
+        hipGraphExecDestroy(s->gexec);
+        s->gexec = NULL;
+    }
     s->d.ctl.T_sim = (double)cfg->T_sim;
     s->d.ctl.out_every = (double)cfg->out_every;
     s->d.ctl.D_h = (double)cfg->D_h;
@@ -916,10 +995,19 @@ struct FcaHipBatch {
     int nwg, cap_wg;
     char *active;
     int *last_step;
+    long long *k_done; /* launch count when a scenario left the list: its
+                        * current controller record is ctrl[k_done & 1] */
     long long k;
     int batch;
     double wall;
+    hipGraphExec_t gexec; /* batch launches over the current workgroup list */
 };
+
+static int launch_batch_step(void *ctx, long long k)
+{
+    FcaHipBatch *b = (FcaHipBatch *)ctx;
+    return fca_k_step_batch(b->tab, b->wgs_dev, b->nwg, b->K, k, b->st);
+}
 
 /* Workgroup list of the active scenarios: {scenario, workgroup} pairs. */
 static int batch_list(FcaHipBatch *b)
@@ -936,6 +1024,10 @@ static int batch_list(FcaHipBatch *b)
         }
     }
     b->nwg = m;
+    if (b->gexec) { /* captured over the previous list */
+        hipGraphExecDestroy(b->gexec);
+        b->gexec = NULL;
+    }
     if (m > 0) {
         HIP_OR(hipMemcpyAsync(b->wgs_dev, b->wgs_host, (size_t)m * 2 * sizeof(int),
                               hipMemcpyHostToDevice, b->st), -2);
@@ -959,6 +1051,9 @@ void fca_hip_batch_close(FcaHipBatch *b)
             fca_hip_sim_close(b->sims[i]);
         }
     }
+    if (b->gexec) {
+        hipGraphExecDestroy(b->gexec);
+    }
     if (b->ctrl_pool) {
         hipFree(b->ctrl_pool);
     }
@@ -980,6 +1075,7 @@ void fca_hip_batch_close(FcaHipBatch *b)
     free(b->sims);
     free(b->active);
     free(b->last_step);
+    free(b->k_done);
     free(b);
 }
 
@@ -1015,12 +1111,13 @@ FcaHipBatch *fca_hip_batch_open(int n, const FcaGrid *const *g, const FcaRunCfg 
     }
     b->n = n;
     b->K = cfg[0]->phys.K;
-    b->batch = (opts && opts->graph_batch > 0) ? opts->graph_batch : 32;
+    b->batch = (opts && opts->graph_batch > 0) ? opts->graph_batch : 30;
     b->sims = (FcaHipSim **)calloc((size_t)n, sizeof(FcaHipSim *));
     b->active = (char *)calloc((size_t)n, 1);
     b->last_step = (int *)calloc((size_t)n, sizeof(int));
+    b->k_done = (long long *)calloc((size_t)n, sizeof(long long));
     devs = (FcaDev **)calloc((size_t)n, sizeof(FcaDev *));
-    if (!b->sims || !b->active || !b->last_step || !devs) {
+    if (!b->sims || !b->active || !b->last_step || !b->k_done || !devs) {
         fail(-2, "out of host memory");
         goto fail_out;
     }
@@ -1106,12 +1203,23 @@ int fca_hip_batch_run(FcaHipBatch *b)
         if (nact == 0) {
             break;
         }
-        for (j = 0; j < b->batch; j++) {
-            if (fca_k_step_batch(b->tab, b->wgs_dev, b->nwg, b->K, b->k, b->st) != 0) {
-                return fail(-2, "batched step launch failed: %s",
-                            hipGetErrorString(hipGetLastError()));
+        if (b->batch % FCA_GRAPH_PERIOD == 0 && b->k % FCA_GRAPH_PERIOD == 0) {
+            if (!b->gexec &&
+                capture_steps(b->st, &b->gexec, b->batch, launch_batch_step, b) != 0) {
+                return -2;
             }
-            b->k++;
+            if (hipGraphLaunch(b->gexec, b->st) != hipSuccess) {
+                return fail(-2, "graph launch failed");
+            }
+            b->k += b->batch;
+        } else {
+            for (j = 0; j < b->batch; j++) {
+                if (launch_batch_step(b, b->k) != 0) {
+                    return fail(-2, "batched step launch failed: %s",
+                                hipGetErrorString(hipGetLastError()));
+                }
+                b->k++;
+            }
         }
         HIP_OR(hipMemcpyAsync(b->h_ctrl, b->ctrl_pool, (size_t)b->n * 2 * sizeof(FcaDevCtrl),
                               hipMemcpyDeviceToHost, b->st), -2);
@@ -1134,6 +1242,7 @@ int fca_hip_batch_run(FcaHipBatch *b)
             b->last_step[i] = c->step;
             if (done) {
                 b->active[i] = 0;
+                b->k_done[i] = b->k;
                 changed = 1;
             }
         }
@@ -1142,10 +1251,17 @@ int fca_hip_batch_run(FcaHipBatch *b)
         }
     }
     for (i = 0; i < b->n; i++) {
-        b->sims[i]->k = b->k;
+        b->sims[i]->k = b->k_done[i];
     }
     b->wall += now_s() - t0;
     return 0;
+}
+
+/* A scenario's session sees its own launch count: the whole batch's while
+ * it is in the list, the count at which it left otherwise. */
+static void batch_sync_k(FcaHipBatch *b, int i)
+{
+    b->sims[i]->k = b->active[i] ? b->k : b->k_done[i];
 }
 
 int fca_hip_batch_read(FcaHipBatch *b, int i, real *h, real *hmax, real *tmax, FcaAcc *acc)
@@ -1153,7 +1269,7 @@ int fca_hip_batch_read(FcaHipBatch *b, int i, real *h, real *hmax, real *tmax, F
     if (!b || i < 0 || i >= b->n) {
         return fail(-1, "bad arguments");
     }
-    b->sims[i]->k = b->k;
+    batch_sync_k(b, i);
     return fca_hip_sim_read(b->sims[i], h, hmax, tmax, acc);
 }
 
@@ -1162,7 +1278,7 @@ int fca_hip_batch_steplog(FcaHipBatch *b, int i, FcaHipStepRec *out, int cap)
     if (!b || i < 0 || i >= b->n) {
         return fail(-1, "bad arguments");
     }
-    b->sims[i]->k = b->k;
+    batch_sync_k(b, i);
     return fca_hip_sim_steplog(b->sims[i], out, cap);
 }
 
@@ -1197,6 +1313,21 @@ int fca_hip_batch_flooded(FcaHipBatch *b, int i, int nthr, const double *thr, lo
         const double m = s->h_red[(size_t)k * 5 + 4];
         *max_depth = m > *max_depth ? m : *max_depth;
     }
+    return 0;
+}
+
+int fca_hip_batch_status(FcaHipBatch *b, int i, int *step, double *t)
+{
+    FcaDevCtrl c;
+    if (!b || i < 0 || i >= b->n || !step || !t) {
+        return fail(-1, "bad arguments");
+    }
+    batch_sync_k(b, i);
+    if (read_ctrl(b->sims[i], &c) != 0) {
+        return -2;
+    }
+    *step = c.step;
+    *t = c.t;
     return 0;
 }
 

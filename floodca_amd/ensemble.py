@@ -92,24 +92,33 @@ def acc_start(prec: str, g: Grid, cfg):
     return acc
 
 
-def run_batched(cases, prec="f32", strip_rows=0, run_wall=None):
+def run_batched(cases, prec="f32", strip_rows=0, run_wall=None, phases=None):
     """One batch over all cases [(grid, cfg)]: returns (seconds, per-scenario
-    (flooded, max depth, steps)). The time covers upload, the run and the
-    per-scenario reductions; run_wall, if a list, receives the seconds of the
-    run alone."""
+    (flooded, max depth, steps)). The time covers upload, the run, the
+    per-scenario reductions and the release; run_wall, if a list, receives
+    the seconds of the run alone; phases, if a dict, the seconds of each
+    phase (open, run, results, close)."""
     t0 = time.perf_counter()
     accs = [acc_start(prec, g, c) for g, c in cases]
     res = []
-    with Batch([g for g, _ in cases], [c for _, c in cases], accs, prec=prec,
-               strip_rows=strip_rows) as b:
+    b = Batch([g for g, _ in cases], [c for _, c in cases], accs, prec=prec,
+              strip_rows=strip_rows)
+    try:
+        t1 = time.perf_counter()
         b.run()
+        t2 = time.perf_counter()
         for i in range(len(cases)):
             fl, md = b.flooded(i, THRESHOLDS)
-            acc = b.read(i, grid=_Sink(cases[i][0]))
-            res.append((fl, md, acc.step))
+            res.append((fl, md, b.steps(i)))
+        t3 = time.perf_counter()
         if run_wall is not None:
             run_wall.append(b.wall)
-    return time.perf_counter() - t0, res
+    finally:
+        b.close()
+    t4 = time.perf_counter()
+    if phases is not None:
+        phases.update(open=t1 - t0, run=t2 - t1, results=t3 - t2, close=t4 - t3)
+    return t4 - t0, res
 
 
 def strip_alone(nr: int, nc: int) -> int:
@@ -131,19 +140,8 @@ def run_alone(cases, prec="f32", strip_rows=None):
         with Batch([g], [c], [acc_start(prec, g, c)], prec=prec, strip_rows=s) as b:
             b.run()
             fl, md = b.flooded(0, THRESHOLDS)
-            acc = b.read(0, grid=_Sink(g))
-            res.append((fl, md, acc.step))
+            res.append((fl, md, b.steps(0)))
     return time.perf_counter() - t0, res
-
-
-class _Sink:
-    """Read target that keeps h/hmax/tmax in fresh arrays, so reading a
-    scenario back does not overwrite its input grid."""
-
-    def __init__(self, g: Grid):
-        self.h = np.empty_like(g.h)
-        self.hmax = np.empty_like(g.hmax)
-        self.tmax = np.empty_like(g.tmax)
 
 
 def _with_tsim(cfg, t_sim):

@@ -65,6 +65,8 @@ class Backend:
                                                 C.POINTER(C.c_double), C.POINTER(C.c_long),
                                                 C.POINTER(C.c_double)]),
             "fca_hip_batch_wall": (C.c_double, [C.c_void_p]),
+            "fca_hip_batch_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                               C.POINTER(C.c_double)]),
             "fca_hip_batch_close": (None, [C.c_void_p]),
         }
         for name, (res, args) in sig.items():
@@ -266,6 +268,14 @@ class Batch:
         buf = (self.a.FcaHipStepRec * max(cap, 1))()
         n = self.b.check(self.b.lib.fca_hip_batch_steplog(self.h, i, buf, cap), "steplog")
         return list(buf)[:n]
+
+    def steps(self, i) -> int:
+        """Steps scenario i has taken (its controller record)."""
+        st = C.c_int()
+        t = C.c_double()
+        self.b.check(self.b.lib.fca_hip_batch_status(self.h, i, C.byref(st), C.byref(t)),
+                     "status")
+        return st.value
 
     def flooded(self, i, thresholds=(0.10, 0.25, 0.50)):
         thr = (C.c_double * len(thresholds))(*thresholds)

@@ -244,7 +244,9 @@ enum { FCA_HIP_FUSED = 0, FCA_HIP_SPLIT = 1 };
 typedef struct FcaHipOpts {
     int kernel;        /* FCA_HIP_FUSED or FCA_HIP_SPLIT */
     int strip_rows;    /* fused: rows walked by one wavefront (default 96) */
-    int graph_batch;   /* steps per captured hipGraph replay (default 32; 1 = no graph) */
+    int graph_batch;   /* launches between controller read-backs, replayed as one
+                        * captured hipGraph when a multiple of 6 (default 30;
+                        * 1 = every launch issued on its own) */
     int record_steps;  /* > 0: keep a per-step log of the first record_steps steps */
     int max_blocks;    /* fused: cap on workgroups, each then walking several strips
                         * (default 0: one strip per wavefront, hardware-dispatched) */
@@ -310,6 +312,8 @@ int fca_hip_batch_steplog(FcaHipBatch *b, int i, FcaHipStepRec *out, int cap);
  * LAND hmax, as the ensemble reports them (ens_common.c:166-188). */
 int fca_hip_batch_flooded(FcaHipBatch *b, int i, int nthr, const double *thr, long *flooded,
                           double *max_depth);
+/* Scenario i's steps taken and simulated time. */
+int fca_hip_batch_status(FcaHipBatch *b, int i, int *step, double *t);
 /* Seconds spent in fca_hip_batch_run so far. */
 double fca_hip_batch_wall(const FcaHipBatch *b);
 void fca_hip_batch_close(FcaHipBatch *b);

@@ -66,3 +66,37 @@ def test_batch_equals_each_scenario_alone(prec, K):
             hm = gb.hmax.astype(np.float64)
             assert fl == [int(np.sum(land & (hm >= t))) for t in (0.10, 0.25, 0.50)], i
             assert md == max(0.0, float(hm[land].max()) if land.any() else 0.0), i
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_graph_replays_equal_launches_issued_one_by_one(prec):
+    """Runs replay captured graphs of 30 step launches between controller
+    read-backs; the same runs with every launch issued on its own
+    (graph_batch=1) end identically, single grid and batch alike."""
+    o = Oracle(prec)
+    cases = scenarios(prec, 4)[:4]
+    res = {}
+    for gb in (1, 0, 12):
+        out = []
+        for g, cfg in cases:
+            ga = g.copy()
+            with Sim(ga, cfg, o.acc_init(ga, cfg), prec=prec, strip_rows=16,
+                     graph_batch=gb) as s:
+                s.run()
+                acc = s.read()
+            out.append((ga.h.copy(), ga.hmax.copy(), acc.step, acc.t, acc.V_rain))
+        grids = [g.copy() for g, _ in cases]
+        with Batch(grids, [c for _, c in cases], [o.acc_init(g, c) for g, (_, c) in
+                                                   zip(grids, cases)],
+                   prec=prec, strip_rows=16, graph_batch=gb) as b:
+            b.run()
+            for i in range(len(cases)):
+                acc = b.read(i)
+                out.append((grids[i].h.copy(), grids[i].hmax.copy(), acc.step, acc.t,
+                            acc.V_rain))
+        res[gb] = out
+    for gb in (0, 12):
+        for x, y in zip(res[1], res[gb]):
+            assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+            assert x[2:] == y[2:]
