@@ -183,6 +183,48 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
     }
 
 
+def sparse_bench(prec: str, size: int = 8192, hours: float = 8.0):
+    """Sparse-start flood front (no rain, one inland inflow cell with a
+    triangular 500 m^3/s hydrograph peaking at 2 h over 8 h, dry start) on a
+    size^2 grid of the same generator: the run to the end of the hydrograph
+    with quiet-tile skipping, and again with every tile walked; the two must
+    end identically."""
+    a = A.abi(prec)
+    g = synth.make_grid(size, size, seed=20261205, dtype=a.dtype)
+    r, c = synth.point_inflow(g, peak_m3s=500.0, dur_h=hours)
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, g.dx)
+    cfg.hyeto, cfg.D_h, cfg.phi = A.FCA_HYETO_TRIANGLE, hours, 0.25
+    cfg.T_sim = hours * 3600.0
+    out = {}
+    for name, flags in (("skip", 0), ("dense", A.FCA_HIP_NO_SKIP)):
+        gg = g.copy()
+        acc = a.FcaAcc()
+        acc.force_day = -1
+        timer = HipTimer()
+        with Sim(gg, cfg, acc, prec=prec, flags=flags) as sim:
+            stream = C.c_void_p(sim.stream)
+            timer.device_sync()
+            timer.start(stream)
+            sim.run()
+            ms = timer.stop(stream)
+            res = sim.read()
+        out[name] = (ms, res.step, gg)
+    same = all(np.array_equal(getattr(out["skip"][2], k), getattr(out["dense"][2], k))
+               for k in ("h", "hmax", "tmax")) and out["skip"][1] == out["dense"][1]
+    gs = out["skip"][2]
+    wet = float(np.mean(gs.hmax[gs.mask == A.FCA_LAND] >= 1e-3))
+    print(f"[bench] sparse start {size}^2: {out['skip'][1]} steps, skip {out['skip'][0]:.1f} ms, "
+          f"dense {out['dense'][0]:.1f} ms, identical={same}, wet fraction {wet:.4f}",
+          file=sys.stderr, flush=True)
+    return {"grid": [size, size], "steps": out["skip"][1], "ms_skip": out["skip"][0],
+            "ms_dense": out["dense"][0], "speedup": out["dense"][0] / out["skip"][0],
+            "results_identical": bool(same), "final_wet_fraction_hmax_1mm": wet,
+            "inflow_cell": [r, c],
+            "workload": f"{size}^2 dry start, one inflow cell, triangular 500 m^3/s hydrograph "
+                        f"peaking at 2 h over {hours:g} h, no rain, FP32, K=4"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +241,8 @@ def main() -> int:
                     help="scenarios of the ensemble measured after the main line (0 = skip)")
     ap.add_argument("--ens-alone", type=int, default=24,
                     help="of which this many are also run one session each")
+    ap.add_argument("--sparse-size", type=int, default=8192,
+                    help="grid side of the sparse-start measurement (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -286,6 +330,8 @@ def main() -> int:
             "bytes_rule": "per LAND cell z,n,cr,h,hmax + mask read, h written; 1 B mask per other cell",
         },
     }
+    if args.sparse_size > 0 and prec == "f32" and args.neigh == 4:
+        line["sparse_start"] = sparse_bench(prec, args.sparse_size)
     if args.ens_scenarios > 0 and prec == "f32" and args.neigh == 4:
         line["ensemble"] = ensemble_bench(g, prec, args.ens_scenarios, args.ens_alone)
     if not args.no_cpu:

@@ -20,7 +20,7 @@ FCA_BND_OPEN, FCA_BND_CLOSED = 0, 1
 FCA_HYETO_TRIANGLE, FCA_HYETO_UNIFORM = 0, 1
 FCA_MAX_FORCE_DAYS = 64
 FCA_HIP_FUSED, FCA_HIP_SPLIT = 0, 1
-FCA_HIP_NO_LAND_FORM = 1
+FCA_HIP_NO_LAND_FORM, FCA_HIP_NO_SKIP = 1, 2
 
 
 def _make(real):

@@ -185,6 +185,20 @@ FCA_HD float rcp_in_range(float b)
 }
 FCA_HD double rcp_in_range(double b) { return 1.0 / b; }
 
+FCA_HDI int same_bits(float a, float b)
+{
+    uint32_t x, y;
+    __builtin_memcpy(&x, &a, 4);
+    __builtin_memcpy(&y, &b, 4);
+    return x == y;
+}
+FCA_HDI int same_bits(double a, double b)
+{
+    uint64_t x, y;
+    __builtin_memcpy(&x, &a, 8);
+    __builtin_memcpy(&y, &b, 8);
+    return x == y;
+}
 FCA_HDI float fabs_r(float x) { return __builtin_fabsf(x); }
 FCA_HDI double fabs_r(double x) { return __builtin_fabs(x); }
 FCA_HDI float fmax_r(float a, float b) { return __builtin_fmaxf(a, b); }

@@ -219,11 +219,16 @@ struct FusedArgs {
     const R *z, *n, *cr;
     R *h0, *h1, *hmax, *tmax;
     const uint8_t *mask;
-    int nr, nc, strip_rows, ngroups, ntiles;
+    int nr, nc, strip_rows, ngroups, ntiles, nstrips;
     PhysK<R> p;
     FcaCtlParams ctl;
     CtlIO io;
     FcaAccSlot *slots;
+    /* quiet-tile skipping (fca_walk.h): flags of steps of each parity,
+     * [2][ntiles], and tiles on which no cell gets rain; skip = 0 walks all */
+    uint8_t *tflags;
+    const uint8_t *tnorain;
+    int skip;
 };
 
 /* Optional occupancy floor for the fused kernel (waves per SIMD). */
@@ -235,7 +240,7 @@ struct FusedArgs {
 
 /* One workgroup of the fused step: wg of nwg workgroups walking a.ntiles
  * tiles of one grid. */
-template <int K8>
+template <int K8, int QT>
 __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
 {
     __shared__ StepVals<R> sh_sv;
@@ -275,11 +280,26 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
         wa.nr = a.nr, wa.nc = a.nc, wa.strip_rows = a.strip_rows;
         wa.p = a.p;
         wa.sv = sv;
+        const uint8_t *prev = a.tflags + (size_t)((sv.step + 1) & 1) * a.ntiles;
+        uint8_t *cur = a.tflags + (size_t)(sv.step & 1) * a.ntiles;
         for (int t = tile; t < a.ntiles; t += W) {
             const int strip = t / a.ngroups, grp = t - strip * a.ngroups;
             const int r0 = strip * a.strip_rows;
             const int r1 = (r0 + a.strip_rows < a.nr) ? r0 + a.strip_rows : a.nr;
-            walk_strip<R, K8>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+            if (QT && tile_skippable(prev, t, a.ngroups, a.nstrips,
+                                     sv.rain_f == (R)0 || (a.tnorain[t] & 1))) {
+                if (lane == 0) {
+                    cur[t] = FCA_TILE_DRY | FCA_TILE_SAME;
+                }
+                continue;
+            }
+            walk_strip<R, K8, QT>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+            if (QT) {
+                const bool wet = lp.any(&WalkLane<R>::qwet), chg = lp.any(&WalkLane<R>::qchg);
+                if (lane == 0) {
+                    cur[t] = (uint8_t)((wet ? 0 : FCA_TILE_DRY) | (chg ? 0 : FCA_TILE_SAME));
+                }
+            }
         }
     }
     walk_acc_finish<R>(lp, a.p);
@@ -312,10 +332,10 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
     }
 }
 
-template <int K8>
+template <int K8, int QT>
 __global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR k_fused(FusedArgs a)
 {
-    fused_body<K8>(a, blockIdx.x, gridDim.x);
+    fused_body<K8, QT>(a, blockIdx.x, gridDim.x);
 }
 
 /* ---- batched fused step ------------------------------------------------------
@@ -346,7 +366,7 @@ __global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR
     a.io.cand_reset = S->cand + (int)((k + 2) % 3);
     a.io.rec = S->rec;
     a.io.rec_cap = S->rec_cap;
-    fused_body<K8>(a, e.y, S->nblocks);
+    fused_body<K8, 0>(a, e.y, S->nblocks);
 }
 
 /* ---- split step ----------------------------------------------------------- */
@@ -610,6 +630,32 @@ __global__ void __launch_bounds__(FCA_BLOCK)
 
 __global__ void k_store_key(Key *dst, Key v) { *dst = v; }
 
+/* Per tile: bit 0 when none of its LAND output cells has cr != 0, bit 1 when
+ * it has LAND output cells. One wavefront per tile. */
+__global__ void __launch_bounds__(64)
+    k_tile_norain(const R *cr, const uint8_t *mask, int nr, int nc, int strip_rows, int ngroups,
+                  uint8_t *out)
+{
+    const int t = blockIdx.x, lane = threadIdx.x;
+    const int strip = t / ngroups, grp = t - strip * ngroups;
+    const int r0 = strip * strip_rows, r1 = (r0 + strip_rows < nr) ? r0 + strip_rows : nr;
+    const int c = grp * FCA_OUTC + lane;
+    int rain = 0, land = 0;
+    if (lane < FCA_OUTC && c < nc) {
+        for (int r = r0; r < r1; r++) {
+            const size_t i = (size_t)r * nc + c;
+            land |= mask[i] == FCA_LAND;
+            rain |= mask[i] == FCA_LAND && !(cr[i] == (R)0);
+        }
+    }
+    rain = __builtin_amdgcn_ballot_w64(rain) != 0;
+    land = __builtin_amdgcn_ballot_w64(land) != 0;
+    if (lane == 0) {
+        /* bit 0: no rain on the tile; bit 1: the tile holds LAND cells */
+        out[t] = (uint8_t)((rain ? 0 : 1) | (land ? 2 : 0));
+    }
+}
+
 __global__ void __launch_bounds__(FCA_BLOCK) k_cr(const R *C, const R *Rday, R *cr, size_t N)
 {
     for (size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x; i < N;
@@ -780,10 +826,12 @@ static FusedArgs fused_args(const FcaDev *d)
     a.z = d->z, a.n = d->n, a.cr = d->cr, a.mask = d->mask;
     a.h0 = d->h[0], a.h1 = d->h[1], a.hmax = d->hmax, a.tmax = d->tmax;
     a.nr = d->nr, a.nc = d->nc, a.strip_rows = d->strip_rows;
-    a.ngroups = d->ngroups, a.ntiles = d->ntiles;
+    a.ngroups = d->ngroups, a.ntiles = d->ntiles, a.nstrips = d->nstrips;
     a.p = phys_of(d);
     a.ctl = d->ctl;
     a.slots = d->slots;
+    a.tflags = d->tflags, a.tnorain = d->tnorain;
+    a.skip = d->skip && d->tflags && d->tnorain;
     return a;
 }
 
@@ -793,10 +841,17 @@ extern "C" int fca_k_step(const FcaDev *d, long long k, hipStream_t st)
     if (d->kernel == FCA_HIP_FUSED) {
         FusedArgs a = fused_args(d);
         a.io = ctl_io(d, k);
+        const int qt = a.skip;
         if (d->K == 8) {
-            hipLaunchKernelGGL(k_fused<1>, dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
+            if (qt) {
+                hipLaunchKernelGGL((k_fused<1, 1>), dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
+            } else {
+                hipLaunchKernelGGL((k_fused<1, 0>), dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
+            }
+        } else if (qt) {
+            hipLaunchKernelGGL((k_fused<0, 1>), dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
         } else {
-            hipLaunchKernelGGL(k_fused<0>, dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
+            hipLaunchKernelGGL((k_fused<0, 0>), dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
         }
     } else {
         SplitArgs a;
@@ -866,6 +921,16 @@ extern "C" int fca_k_step_batch(const void *tab, const void *wgs, int nwg, int K
         hipLaunchKernelGGL(k_fused_batch<0>, dim3(nwg), dim3(FCA_BLOCK), 0, st,
                            (const ScenDev *)tab, (const int2 *)wgs, k);
     }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fca_k_tile_norain(const FcaDev *d, hipStream_t st)
+{
+    if (!d->tnorain || d->ntiles <= 0) {
+        return 0;
+    }
+    hipLaunchKernelGGL(k_tile_norain, dim3(d->ntiles), dim3(64), 0, st, d->cr, d->mask, d->nr,
+                       d->nc, d->strip_rows, d->ngroups, d->tnorain);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -37,6 +37,9 @@ typedef struct FcaDev {
     int land_form;
     /* fused geometry; max_blocks caps the persistent grid (0 = one wave per tile) */
     int strip_rows, ngroups, nstrips, ntiles, nblocks, max_blocks;
+    /* quiet-tile skipping: flags [2][ntiles], rain-free tiles [ntiles], on/off */
+    uint8_t *tflags, *tnorain;
+    int skip;
     double *red;            /* scratch of the volume / ensemble reductions */
     int nred;
 } FcaDev;
@@ -56,6 +59,8 @@ void fca_k_geometry(FcaDev *d);
 int fca_k_cand_init(const FcaDev *d, int hbuf, int slot, hipStream_t st);
 /* One step launch with launch index k (ctrl parity, candidate rotation). */
 int fca_k_step(const FcaDev *d, long long k, hipStream_t st);
+/* Rain-free tiles of the current cr (for quiet-tile skipping). */
+int fca_k_tile_norain(const FcaDev *d, hipStream_t st);
 /* cr = C*Rday on every cell. */
 int fca_k_cr(const FcaDev *d, hipStream_t st);
 /* Sets candidate slot `slot` to +inf. */

@@ -87,6 +87,9 @@ struct WalkLane {
     /* stores of row r-3 deferred to the next row's memory block: bit 0 h,
      * bit 1 hmax and tmax */
     int stq;
+    /* this tile's output cells: some depth >= tau_h after the step (qwet),
+     * some h, hmax or tmax changed by the step (qchg) */
+    int qwet, qchg;
     /* accumulators of this lane's output cells */
     double v_rain, clamp, v_sea, v_bnd;
     long long cap;
@@ -115,6 +118,38 @@ struct WalkRow {
     int r, r_out0, r_out1, c_first;
     int edge_cols; /* the wavefront holds column 0 or nc-1 */
 };
+
+/* Quiet tiles. A tile whose output cells and their neighbours all hold less
+ * than tau_h of water, and on which no rain falls this step, is left exactly
+ * as it is by the step: every face touching its cells is dead (hf < tau_h for
+ * two such cells, for a sea or open face too), so net = 0 and
+ * h + (dt*(0/A) + 0) = h, no hmax rises, no balance term or time step comes
+ * from it. Such a tile need not be walked, provided the depth buffer the
+ * step would write already holds its cells' values, i.e. the tile did not
+ * change in the previous step either. Per tile and step, bit 0: all output
+ * depths < tau_h after the step; bit 1: the step changed none of them. */
+#define FCA_TILE_DRY 1
+#define FCA_TILE_SAME 2
+
+/* Whether tile t may skip this step given the flags of the previous step
+ * (tiles of ngroups columns, nstrips rows) and whether no rain falls on it. */
+FCA_HDI int tile_skippable(const uint8_t *prev, int t, int ngroups, int nstrips, int no_rain)
+{
+    const int st = t / ngroups, gr = t - st * ngroups;
+    if (!no_rain || (prev[t] & (FCA_TILE_DRY | FCA_TILE_SAME)) != (FCA_TILE_DRY | FCA_TILE_SAME)) {
+        return 0;
+    }
+    for (int ds = -1; ds <= 1; ds++) {
+        for (int dg = -1; dg <= 1; dg++) {
+            const int s2 = st + ds, g2 = gr + dg;
+            if (s2 >= 0 && s2 < nstrips && g2 >= 0 && g2 < ngroups &&
+                !(prev[s2 * ngroups + g2] & FCA_TILE_DRY)) {
+                return 0;
+            }
+        }
+    }
+    return 1;
+}
 
 /* Upper bound test for stage D: whether a wet LAND cell's exact speed
  * (cell_speed) might exceed thr > 0, from its depth h, roughness n and an
@@ -185,7 +220,7 @@ FCA_HDI OpenFaces<R> open_faces(R z, R h, R n, int m, int c, int q, int nr, int 
 /* One row of the walk after row r's inputs were taken into the ring.
  * LANDF: rows r-4..r are LAND on every lane, 1 <= r-4, r <= nr-2 and no lane
  * holds column 0 or nc-1. */
-template <class R, int K8, int LANDF, class LP>
+template <class R, int K8, int LANDF, int QT, class LP>
 FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
 {
     typedef WalkLane<R> L;
@@ -467,6 +502,10 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                     s.v_rain += (double)A * (double)rain;
                     s.clamp += clamp_here;
                     s.stq = (hnew > s.phm) ? 3 : 1;
+                    if (QT) {
+                        s.qwet |= !(hnew < p.tau_h);
+                        s.qchg |= !same_bits(hnew, s.h2) || hnew > s.phm;
+                    }
                 }
             }
         });
@@ -577,8 +616,10 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
 }
 
 /* Walks one strip for one wavefront. strip covers output rows
- * [r_out0, r_out1) and columns [c_first + FCA_HALO, c_first + FCA_HALO + 58). */
-template <class R, int K8, class LP>
+ * [r_out0, r_out1) and columns [c_first + FCA_HALO, c_first + FCA_HALO + 58).
+ * QT: also record whether the strip's cells end wet or changed (qwet, qchg)
+ * for quiet-tile skipping. */
+template <class R, int K8, int QT, class LP>
 FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int c_first)
 {
     typedef WalkLane<R> L;
@@ -610,6 +651,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         s.s1 = s.s2 = s.s3 = (R)1;
         s.pcr = s.phm = s.pcr1 = s.phm1 = (R)0;
         s.stq = 0;
+        s.qwet = s.qchg = 0;
         s.pz = s.ph = s.pn = (R)0;
         s.pm1 = s.pm2 = FCA_NONE;
         if (s.colok && rb < nr) {
@@ -693,7 +735,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         if (a.p.fast && cols_inner && (land_rows & 0x1Fu) == 0x1Fu && r - 4 >= 1 &&
             r <= nr - 2) {
             lp.count_form(1);
-            walk_row<R, K8, 1>(lp, a, w);
+            walk_row<R, K8, 1, QT>(lp, a, w);
         } else if ((void_rows & 0xFu) == 0xFu) {
             /* rows r-3..r hold no LAND cell: every face touching rows r-1
              * and r is dead, the limiter of row r-1 is 1, row r-2 keeps its
@@ -707,7 +749,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             });
         } else {
             lp.count_form(0);
-            walk_row<R, K8, 0>(lp, a, w);
+            walk_row<R, K8, 0, QT>(lp, a, w);
         }
 
         /* ---- rotate the ring ----------------------------------------------- */

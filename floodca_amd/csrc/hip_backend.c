@@ -67,6 +67,7 @@ struct FcaHipSim {
     int n_ok;              /* every LAND roughness in the LAND form's range */
     int no_land_form;      /* FCA_HIP_NO_LAND_FORM was asked for */
     int own_stream;        /* st was created for this session */
+    int skip_wanted;       /* quiet-tile skipping not switched off */
     hipGraphExec_t gexec;  /* glen step launches, captured once */
     int glen;
 };
@@ -112,7 +113,7 @@ static void dev_free(FcaDev *d)
 {
     void *ptrs[] = {d->z, d->n, d->C, d->Rday, d->cr, d->h[0], d->h[1], d->hmax, d->tmax,
                     d->mask, d->qx, d->qy, d->qd1, d->qd2, d->qo, d->s, d->sv, d->ctrl,
-                    d->cand, d->slots, d->rec, d->red};
+                    d->cand, d->slots, d->rec, d->red, d->tflags, d->tnorain};
     size_t i;
     for (i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) {
         if (ptrs[i]) {
@@ -283,6 +284,12 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
                                            sizeof(FcaHipStepRec)), -2);
     d->nred = NRED * 5;
     HIP_OR(hipMalloc((void **)&d->red, (size_t)d->nred * sizeof(double)), -2);
+    if (d->kernel == FCA_HIP_FUSED) {
+        s->skip_wanted = !(o && (o->flags & FCA_HIP_NO_SKIP));
+        d->skip = 0;
+        HIP_OR(hipMalloc((void **)&d->tflags, (size_t)2 * d->ntiles), -2);
+        HIP_OR(hipMalloc((void **)&d->tnorain, (size_t)d->ntiles), -2);
+    }
     if (d->kernel == FCA_HIP_SPLIT) {
         size_t nqx = (size_t)d->nr * (size_t)(d->nc - 1), nqy = (size_t)(d->nr - 1) * d->nc;
         size_t nqd = (size_t)(d->nr - 1) * (size_t)(d->nc - 1);
@@ -343,7 +350,36 @@ static int sim_upload_state(FcaHipSim *s, const FcaGrid *g, const FcaAcc *acc)
     c.dt_base = (double)acc->dt_base;
     HIP_OR(hipMemcpy(d->ctrl, &c, sizeof(c), hipMemcpyHostToDevice), -2);
     HIP_OR(hipMemset(d->slots, 0, (size_t)d->nslots * sizeof(FcaAccSlot)), -2);
-    if (fca_k_cr(d, s->st) != 0 || fca_k_cand_init(d, 0, 0, s->st) != 0 ||
+    if (d->tflags) { /* no tile is known quiet before its first walk */
+        HIP_OR(hipMemset(d->tflags, 0, (size_t)2 * d->ntiles), -2);
+    }
+    if (d->tflags && s->skip_wanted) {
+        /* skipping can only help where a LAND tile gets no rain or the
+         * hyetograph can fall to zero; otherwise the kernel without the
+         * quiet-tile bookkeeping runs */
+        int can = s->cfg->hyeto != FCA_HYETO_UNIFORM || s->cfg->n_force_days > 0;
+        if (!can) {
+            uint8_t *tn = (uint8_t *)malloc((size_t)d->ntiles);
+            int t;
+            if (!tn) {
+                return fail(-2, "out of host memory");
+            }
+            if (fca_k_cr(d, s->st) != 0 || fca_k_tile_norain(d, s->st) != 0 ||
+                hipMemcpyAsync(tn, d->tnorain, (size_t)d->ntiles, hipMemcpyDeviceToHost,
+                               s->st) != hipSuccess ||
+                hipStreamSynchronize(s->st) != hipSuccess) {
+                free(tn);
+                return fail(-2, "rain-free tile map failed");
+            }
+            for (t = 0; t < d->ntiles && !can; t++) {
+                can = (tn[t] & 3) == 3;
+            }
+            free(tn);
+        }
+        d->skip = can;
+    }
+    if (fca_k_cr(d, s->st) != 0 || fca_k_tile_norain(d, s->st) != 0 ||
+        fca_k_cand_init(d, 0, 0, s->st) != 0 ||
         fca_k_cand_reset(d, 1, s->st) != 0 || fca_k_cand_reset(d, 2, s->st) != 0) {
         return fail(-2, "kernel launch failed during upload");
     }
@@ -519,7 +555,7 @@ static int reload_day(FcaHipSim *s, int day)
     t0 = now_s();
     HIP_OR(hipMemcpyAsync(d->Rday, s->h_stage, s->N * sizeof(real), hipMemcpyHostToDevice,
                           s->st), -2);
-    if (fca_k_cr(d, s->st) != 0) {
+    if (fca_k_cr(d, s->st) != 0 || fca_k_tile_norain(d, s->st) != 0) {
         return fail(-2, "cr launch failed");
     }
     HIP_OR(hipStreamSynchronize(s->st), -2);

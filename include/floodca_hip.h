@@ -255,7 +255,10 @@ typedef struct FcaHipOpts {
     int reserved[2];
 } FcaHipOpts;
 
-enum { FCA_HIP_NO_LAND_FORM = 1 };
+/* flags: FCA_HIP_NO_LAND_FORM as above; FCA_HIP_NO_SKIP walks every tile
+ * every step (by default a tile that is dry, rain-free and unchanged with
+ * its neighbours is skipped, which leaves every result as it is). */
+enum { FCA_HIP_NO_LAND_FORM = 1, FCA_HIP_NO_SKIP = 2 };
 
 /* One executed step, as the controller saw it (fca_step, model.c:363-382). */
 typedef struct FcaHipStepRec {

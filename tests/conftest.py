@@ -85,10 +85,12 @@ def emu_lib(prec):
     return L
 
 
-def emu_run(grid, cfg, acc, prec, strip_rows=0, cap=100000, land_form=True, forms=None):
+def emu_run(grid, cfg, acc, prec, strip_rows=0, cap=100000, land_form=True, forms=None,
+            skip=True, skipped=None):
     """Runs the emulated fused walk; land_form=False keeps every row in the
-    general form. forms, if a list, receives the rows walked in the general,
-    the LAND and the no-LAND form."""
+    general form, skip=False walks every tile. forms, if a list, receives the
+    rows walked in the general, the LAND and the no-LAND form; skipped, if a
+    list, the number of quiet tiles skipped."""
     import ctypes as C
 
     a = A.abi(prec)
@@ -97,11 +99,16 @@ def emu_run(grid, cfg, acc, prec, strip_rows=0, cap=100000, land_form=True, form
     rec = (a.FcaHipStepRec * cap)()
     n = C.c_int()
     L.emu_set_land_form(-1 if land_form else 0)
+    L.emu_set_skip(1 if skip else 0)
     try:
         assert L.emu_run(C.byref(g), C.byref(cfg), C.byref(acc), strip_rows, rec, cap,
                          C.byref(n)) == 0
     finally:
         L.emu_set_land_form(-1)
+        L.emu_set_skip(1)
+    if skipped is not None:
+        L.emu_skipped_tiles.restype = C.c_long
+        skipped[:] = [L.emu_skipped_tiles()]
     if forms is not None:
         gen, land, noland = C.c_long(), C.c_long(), C.c_long()
         L.emu_form_rows(C.byref(gen), C.byref(land), C.byref(noland))

@@ -101,6 +101,41 @@ extern "C" void emu_set_land_form(int v)
     emu_land_form = v;
 }
 
+/* 1: skip quiet tiles as the kernel does (the default); 0: walk every tile. */
+static int emu_skip = 1;
+static long emu_skipped;
+extern "C" void emu_set_skip(int v)
+{
+    emu_skip = v;
+}
+/* Tiles skipped by the last emu_run. */
+extern "C" long emu_skipped_tiles(void)
+{
+    return emu_skipped;
+}
+
+/* The kernel's rain-free tile map (k_tile_norain). */
+static void tile_norain(const R *cr, const uint8_t *mask, int nr, int nc, int strip_rows,
+                        int ngroups, int ntiles, uint8_t *out)
+{
+    for (int t = 0; t < ntiles; t++) {
+        const int strip = t / ngroups, grp = t % ngroups;
+        const int r0 = strip * strip_rows, r1 = (r0 + strip_rows < nr) ? r0 + strip_rows : nr;
+        int rain = 0, land = 0;
+        for (int r = r0; r < r1; r++) {
+            for (int l = 0; l < FCA_OUTC; l++) {
+                const int c = grp * FCA_OUTC + l;
+                if (c < nc) {
+                    const size_t i = (size_t)r * nc + c;
+                    land |= mask[i] == FCA_LAND;
+                    rain |= mask[i] == FCA_LAND && !(cr[i] == (R)0);
+                }
+            }
+        }
+        out[t] = (uint8_t)((rain ? 0 : 1) | (land ? 2 : 0));
+    }
+}
+
 /* Rows walked in the general, the LAND and the no-LAND form by the last
  * emu_run. */
 static long emu_rows[3];
@@ -192,6 +227,14 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
     p.fast = emu_land_form == 0 ? 0 : fca_land_form_ok(&cfg->phys, g->n, g->mask, N);
     ngroups = (nc + FCA_OUTC - 1) / FCA_OUTC;
     nstrips = (nr + strip_rows - 1) / strip_rows;
+    uint8_t *tflags = (uint8_t *)calloc((size_t)2 * ngroups * nstrips, 1);
+    uint8_t *tnorain = (uint8_t *)calloc((size_t)ngroups * nstrips, 1);
+    if (!tflags || !tnorain) {
+        free(tflags), free(tnorain), free(h[0]), free(h[1]), free(cr);
+        return -1;
+    }
+    tile_norain(cr, g->mask, nr, nc, strip_rows, ngroups, ngroups * nstrips, tnorain);
+    emu_skipped = 0;
 
     for (;;) {
         StepVals<R> sv;
@@ -208,6 +251,7 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
             for (i = 0; i < N; i++) {
                 cr[i] = g->C[i] * g->Rday[i];
             }
+            tile_norain(cr, g->mask, nr, nc, strip_rows, ngroups, ngroups * nstrips, tnorain);
             c.force_day = c.want_day;
             c.state = FCA_CTL_RUN;
             continue;
@@ -244,12 +288,27 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
                 const int strip = t / ngroups, grp = t % ngroups;
                 const int r0 = strip * strip_rows;
                 const int r1 = (r0 + strip_rows < nr) ? r0 + strip_rows : nr;
+                const int ntl = ngroups * nstrips;
+                const uint8_t *prev = tflags + (size_t)((c.step + 1) & 1) * ntl;
+                uint8_t *cur = tflags + (size_t)(c.step & 1) * ntl;
                 walk_acc_init<R>(lp);
-                if (K8) {
-                    walk_strip<R, 1>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
-                } else {
-                    walk_strip<R, 0>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                if (emu_skip && tile_skippable(prev, t, ngroups, nstrips,
+                                               sv.rain_f == (R)0 || (tnorain[t] & 1))) {
+                    cur[t] = FCA_TILE_DRY | FCA_TILE_SAME;
+                    emu_skipped++;
+                    continue;
                 }
+                if (K8 && emu_skip) {
+                    walk_strip<R, 1, 1>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                } else if (K8) {
+                    walk_strip<R, 1, 0>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                } else if (emu_skip) {
+                    walk_strip<R, 0, 1>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                } else {
+                    walk_strip<R, 0, 0>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                }
+                cur[t] = (uint8_t)((lp.any(&WalkLane<R>::qwet) ? 0 : FCA_TILE_DRY) |
+                                   (lp.any(&WalkLane<R>::qchg) ? 0 : FCA_TILE_SAME));
                 walk_acc_finish<R>(lp, p);
                 emu_rows[0] = lp.form_rows[0], emu_rows[1] = lp.form_rows[1];
                 emu_rows[2] = lp.form_rows[2];
@@ -286,6 +345,6 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
     if (nrec) {
         *nrec = nrc;
     }
-    free(h[0]), free(h[1]), free(cr);
+    free(h[0]), free(h[1]), free(cr), free(tflags), free(tnorain);
     return 0;
 }

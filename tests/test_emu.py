@@ -86,3 +86,30 @@ def test_all_three_row_forms_match_oracle(prec, K):
     assert [r.dt for r in log_o] == [r.dt for r in log_e]
     assert acc_o.cap_active == acc_e.cap_active
     assert acc_o.V_sea == pytest.approx(acc_e.V_sea, rel=1e-12, abs=1e-9)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("K", [4, 8])
+@pytest.mark.parametrize("bnd", [A.FCA_BND_OPEN, A.FCA_BND_CLOSED])
+def test_quiet_tiles_skip_without_changing_results(prec, K, bnd):
+    """A sparse start (dry grid, one inflow cell, a second short rain): most
+    tiles are quiet, are skipped, and the run is the oracle's bit for bit,
+    and the same with skipping off."""
+    from floodca_amd import synth
+    a, g, cfg = make_case(90, 250, prec=prec, K=K, boundary=bnd, T_sim=3000.0, h0=0.0,
+                          seed=17, land_frac=0.8)
+    synth.point_inflow(g, peak_m3s=50.0, dur_h=1.0)
+    cfg.hyeto, cfg.D_h, cfg.phi = A.FCA_HYETO_TRIANGLE, 1.0, 0.25
+    o = Oracle(prec)
+    go, ge, gn = g.copy(), g.copy(), g.copy()
+    acc_o, log_o = o.run(go, cfg, log_cap=100000)
+    sk = []
+    acc_e, log_e = emu_run(ge, cfg, o.acc_init(ge, cfg), prec, strip_rows=8, skipped=sk)
+    acc_n, log_n = emu_run(gn, cfg, o.acc_init(gn, cfg), prec, strip_rows=8, skip=False)
+    assert sk[0] > 0
+    for k in ("h", "hmax", "tmax"):
+        assert np.array_equal(getattr(go, k), getattr(ge, k)), k
+        assert np.array_equal(getattr(gn, k), getattr(ge, k)), k
+    assert [r.dt for r in log_o] == [r.dt for r in log_e] == [r.dt for r in log_n]
+    assert acc_o.cap_active == acc_e.cap_active
+    assert acc_e.V_rain == acc_n.V_rain and acc_e.V_bnd == acc_n.V_bnd
