@@ -225,6 +225,61 @@ def sparse_bench(prec: str, size: int = 8192, hours: float = 8.0):
                         f"peaking at 2 h over {hours:g} h, no rain, FP32, K=4"}
 
 
+def forward_flops(cfg) -> int:
+    """Multiply-adds x 2 of one sample's forward pass (every GEMM's M*N*K)."""
+    from floodca_amd import nn
+    total = 0
+    for ly in nn.layers(cfg):
+        total += ly.hout * ly.hout * ly.cout * ly.k
+    return 2 * total
+
+
+def surrogate_bench(base, args):
+    """The surrogate-screened ensemble (floodca_amd.surrogate.run_pipeline)
+    on crops of the benchmark DEM, and the forward pass timed alone with HIP
+    events on its stream."""
+    import torch
+    from floodca_amd import nn
+    from floodca_amd import surrogate as SG
+    pc = SG.PipelineCfg(n_train=args.sur_train, n_val=max(64, args.sur_train // 16),
+                        n_cand=args.sur_cand, epochs=args.sur_epochs, n_alone=args.sur_alone)
+    res = SG.run_pipeline(base, pc, log=lambda m: print(f"[bench] {m}", file=sys.stderr,
+                                                       flush=True))
+    o = res.pop("_objects")
+    net = o["net"]
+    B = pc.screen_batch
+    q = pc.R // 4
+    X = torch.zeros((B, q, q, 64), dtype=torch.uint16, device="cuda")
+    S = torch.zeros((B, 4), dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        net.forward(X, S, B)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    reps = 20
+    for _ in range(reps):
+        net.forward(X, S, B)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    fl = forward_flops(net.cfg)
+    tflops = fl * B / (ms * 1e-3) / 1e12
+    res["forward_ms_per_batch"] = ms
+    res["forward_batch"] = B
+    res["forward_gflop_per_sample"] = fl / 1e9
+    res["forward_tflops"] = tflops
+    res["forward_mfma_frac_of_bf16_peak"] = tflops / 2500.0
+    res["workload"] = (f"{pc.n_train} CA runs ({pc.n_val} held out) and {pc.n_cand} candidates "
+                       f"from the ensemble generator on crops of the {base.shape[0]}^2 DEM, "
+                       f"side U{list(pc.side)}, storm D U{list(pc.dur)} h; network "
+                       f"{net.n} parameters at {pc.R}^2, BF16 on MFMA, FP32 accumulation; "
+                       f"CA reruns of the top {pc.frac_area:.0%} by predicted flooded area "
+                       f"+ {pc.frac_unc:.0%} by uncertainty")
+    print(f"[bench] surrogate forward: {ms:.2f} ms per {B} samples, {tflops:.1f} TFLOP/s",
+          file=sys.stderr, flush=True)
+    return {k: (round(v, 6) if isinstance(v, float) else v) for k, v in res.items()}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +298,13 @@ def main() -> int:
                     help="of which this many are also run one session each")
     ap.add_argument("--sparse-size", type=int, default=8192,
                     help="grid side of the sparse-start measurement (0 = skip)")
+    ap.add_argument("--surrogate", choices=("off", "on"), default="off",
+                    help="run the surrogate-screened ensemble after the main line")
+    ap.add_argument("--sur-train", type=int, default=4096, help="CA runs of its dataset")
+    ap.add_argument("--sur-cand", type=int, default=100000, help="candidates it screens")
+    ap.add_argument("--sur-epochs", type=int, default=30)
+    ap.add_argument("--sur-alone", type=int, default=64,
+                    help="candidates timed one session each (the all-CA baseline)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -334,6 +396,8 @@ def main() -> int:
         line["sparse_start"] = sparse_bench(prec, args.sparse_size)
     if args.ens_scenarios > 0 and prec == "f32" and args.neigh == 4:
         line["ensemble"] = ensemble_bench(g, prec, args.ens_scenarios, args.ens_alone)
+    if args.surrogate == "on" and prec == "f32" and args.neigh == 4:
+        line["surrogate"] = surrogate_bench(g, args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds, prec)
     print(json.dumps(line), flush=True)

@@ -2,6 +2,7 @@
 
 * floodca_amd/libfloodca_hip.so, libfloodca_hip_f64.so -- the product: the HIP
   kernels (hipcc, gfx950) and the C host (gcc), one library per precision.
+* floodca_amd/libfca_nn.so -- the surrogate network's kernels (hipcc, gfx950).
 * tests/emu/libwalk_emu_{f32,f64}.so -- host emulation of the fused walk for the
   CPU test suite (g++), never loaded by the product.
 * oracle/ -- the checker (oracle/build_oracle.sh).
@@ -69,6 +70,20 @@ def build_hip(prec: str, extra=(), out: str | None = None, tag: str = "") -> str
     return out
 
 
+def nn_lib_path() -> str:
+    """The in-tree surrogate kernel library; FCA_NN_LIB overrides it."""
+    return os.environ.get("FCA_NN_LIB") or os.path.join(PKG, "libfca_nn.so")
+
+
+def build_nn(_=None) -> str:
+    out = nn_lib_path()
+    tmp = out + ".tmp"
+    hipcc = os.path.join(ROCM, "bin", "hipcc")
+    _run([hipcc, *HIP_FLAGS, "-shared", os.path.join(CSRC, "fca_nn.hip"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_emu(prec: str) -> str:
     defs, _ = PRECS[prec]
     out = os.path.join(ROOT, "tests", "emu", f"libwalk_emu_{prec}.so")
@@ -95,8 +110,9 @@ def build_oracle() -> None:
 def build_all(verbose: bool = True) -> list[str]:
     if shutil.which(os.path.join(ROCM, "bin", "hipcc")) is None:
         raise RuntimeError(f"hipcc not found under {ROCM}")
-    jobs = [(build_hip, "f32"), (build_hip, "f64"), (build_emu, "f32"), (build_emu, "f64")]
-    with ThreadPoolExecutor(max_workers=4) as ex:
+    jobs = [(build_hip, "f32"), (build_hip, "f64"), (build_emu, "f32"), (build_emu, "f64"),
+            (build_nn, None)]
+    with ThreadPoolExecutor(max_workers=5) as ex:
         futs = [ex.submit(f, p) for f, p in jobs] + [ex.submit(build_oracle),
                                                      ex.submit(build_mathcheck)]
         outs = [f.result() for f in futs]

@@ -1367,6 +1367,19 @@ int fca_hip_batch_status(FcaHipBatch *b, int i, int *step, double *t)
     return 0;
 }
 
+int fca_hip_batch_device_fields(FcaHipBatch *b, int i, void **hmax, void **mask, int *nr,
+                                int *nc)
+{
+    if (!b || i < 0 || i >= b->n || !hmax || !mask || !nr || !nc) {
+        return fail(-1, "bad arguments");
+    }
+    *hmax = b->sims[i]->d.hmax;
+    *mask = b->sims[i]->d.mask;
+    *nr = b->sims[i]->d.nr;
+    *nc = b->sims[i]->d.nc;
+    return 0;
+}
+
 double fca_hip_batch_wall(const FcaHipBatch *b)
 {
     return b ? b->wall : 0.0;

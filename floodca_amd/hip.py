@@ -65,6 +65,10 @@ class Backend:
                                                 C.POINTER(C.c_double), C.POINTER(C.c_long),
                                                 C.POINTER(C.c_double)]),
             "fca_hip_batch_wall": (C.c_double, [C.c_void_p]),
+            "fca_hip_batch_device_fields": (C.c_int, [C.c_void_p, C.c_int,
+                                                      C.POINTER(C.c_void_p),
+                                                      C.POINTER(C.c_void_p),
+                                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "fca_hip_batch_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                                C.POINTER(C.c_double)]),
             "fca_hip_batch_close": (None, [C.c_void_p]),
@@ -276,6 +280,14 @@ class Batch:
         self.b.check(self.b.lib.fca_hip_batch_status(self.h, i, C.byref(st), C.byref(t)),
                      "status")
         return st.value
+
+    def device_fields(self, i):
+        """(hmax, mask, nr, nc): device addresses of scenario i's fields."""
+        hm, mk, nr, nc = C.c_void_p(), C.c_void_p(), C.c_int(), C.c_int()
+        self.b.check(self.b.lib.fca_hip_batch_device_fields(self.h, i, C.byref(hm), C.byref(mk),
+                                                            C.byref(nr), C.byref(nc)),
+                     "device fields")
+        return hm.value, mk.value, nr.value, nc.value
 
     def flooded(self, i, thresholds=(0.10, 0.25, 0.50)):
         thr = (C.c_double * len(thresholds))(*thresholds)

@@ -319,6 +319,13 @@ int fca_hip_batch_flooded(FcaHipBatch *b, int i, int nthr, const double *thr, lo
 int fca_hip_batch_status(FcaHipBatch *b, int i, int *step, double *t);
 /* Seconds spent in fca_hip_batch_run so far. */
 double fca_hip_batch_wall(const FcaHipBatch *b);
+/* Device addresses of scenario i's hmax (real, nr x nc row-major) and mask
+ * (uint8), valid until fca_hip_batch_close and complete once
+ * fca_hip_batch_run has returned: for consumers that reduce the product on
+ * the device without a copy to the host (no counterpart in the reference,
+ * whose workers copy hmax back, src/hybrid/ens_common.c:166-188). */
+int fca_hip_batch_device_fields(FcaHipBatch *b, int i, void **hmax, void **mask, int *nr,
+                                int *nc);
 void fca_hip_batch_close(FcaHipBatch *b);
 
 #ifdef __cplusplus
