@@ -54,14 +54,15 @@ typedef struct FcaAccSlot {
 /* Whether a run admits the LAND form of the fused walk. The thresholds keep
  * the arguments of the face's divisions, square root and hf^(2/3) inside the
  * ranges where the device shortcuts need no range test (|dL| >= tau_l >=
- * 2^-80, RN(tau_l/d) >= 2^-96 for the longest face, hf >= tau_h >= 2^-15),
- * and every LAND roughness in [1e-12, 2^100] lets the face roughness go
- * without its clamp and its reciprocal without a range test. */
+ * 2^-80, RN(tau_l/d) >= 2^-96 for the longest face, hf >= tau_h >= 2^-15,
+ * g*hf >= 2^-100 with g in [2^-80, 2^80]), and every LAND roughness in
+ * [1e-12, 2^100] lets the face roughness go without its clamp and its
+ * reciprocal without a range test. */
 static inline int fca_land_phys_ok(const FcaPhys *ph)
 {
     const real dx = ph->dx, dmax = (ph->K == 8) ? (real)sqrt(2.0) * dx : dx;
     return ph->tau_l >= (real)0x1p-80 && ph->tau_l / dmax >= (real)0x1p-96 &&
-           ph->tau_h >= (real)0x1p-15;
+           ph->tau_h >= (real)0x1p-15 && ph->g >= (real)0x1p-80 && ph->g <= (real)0x1p80;
 }
 static inline int fca_land_n_ok(const real *n, const uint8_t *mask, size_t N)
 {
@@ -320,6 +321,29 @@ FCA_HD double cbrt_sq(double hf)
     return cbrt(sq);
 }
 
+/* cbrt_sq for hf >= 2^-15 (the LAND form's hf >= tau_h): the fast path's
+ * lower range test is dropped. */
+FCA_HD float cbrt_sq_big(float hf)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double sq = (double)hf * (double)hf;
+    if (hf <= FCA_CBRT_HI) {
+        const double y1 = cbrt_sq_newton(hf, sq);
+        uint64_t bits;
+        __builtin_memcpy(&bits, &y1, 8);
+        const int lo = (int)((uint32_t)bits & 0x1FFFFFFFu) - 0x10000000;
+        if (lo > FCA_CBRT_GUARD || lo < -FCA_CBRT_GUARD) {
+            return (float)y1;
+        }
+        return cbrt_sq_fix((float)y1, sq);
+    }
+    return cbrt_sq_fix((float)cbrt(sq), sq);
+#else
+    return cbrt_sq(hf);
+#endif
+}
+FCA_HD double cbrt_sq_big(double hf) { return cbrt_sq(hf); }
+
 FCA_HD float sqrt_via_double(float x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -406,12 +430,13 @@ FCA_HDI R level_cap(R qmag, R adL, R dt, R den, R rden, const PhysK<R> &p, int &
  * 108-111). The exact critical speed is only needed when v is not clearly
  * below it: fl(v*v) < fl(fl(g*hf)*(1-2^-16)) implies v < sqrt(g*hf)*(1-2^-18)
  * < vcr, as each rounding is within 2^-24 (2^-53 in double); g*hf >= 2^-100
- * keeps the comparison clear of underflow. */
-template <class R>
+ * keeps the comparison clear of underflow. GH_OK: the caller knows
+ * g*hf >= 2^-100 (the LAND form: hf >= tau_h >= 2^-15 and g >= 2^-80). */
+template <class R, int GH_OK = 0>
 FCA_HDI R crit_cap(R v, R hf, const PhysK<R> &p)
 {
     const R gh = p.g * hf;
-    if (!(v * v < gh * ((R)1 - (R)0x1p-16) && gh >= (R)0x1p-100)) {
+    if (!(v * v < gh * ((R)1 - (R)0x1p-16) && (GH_OK || gh >= (R)0x1p-100))) {
         FCA_COLD_PATH();
         const R vcr = crit_speed(p.g, hf);
         if (v > vcr) {
@@ -481,8 +506,8 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
     cap = 0;
     if (!((adL < p.tau_l) | (hf < p.tau_h))) {
         const R nf = (R)0.5 * (na + nb);
-        R v = rcp_in_range(nf) * cbrt_sq(hf) * sqrt_in_range(div_c_big(adL, d, rd));
-        v = crit_cap<R>(v, hf, p);
+        R v = rcp_in_range(nf) * cbrt_sq_big(hf) * sqrt_in_range(div_c_big(adL, d, rd));
+        v = crit_cap<R, 1>(v, hf, p);
         qmag = v * hf * p.w;
         qmag = level_cap<R>(qmag, adL, dt, den, rden, p, cap);
         qmag = (dL > (R)0) ? qmag : -qmag;

@@ -94,6 +94,7 @@ struct WalkLane {
     double v_rain, clamp, v_sea, v_bnd;
     long long cap;
     R dtc, vmax, vthr;
+    R lthr, lgg; /* log2 forms of vthr and sqrt(g) for speed_may_exceed_fast */
     /* stage D: largest |level difference| per face length, and whether the
      * exact speed is needed */
     R dax, dad;
@@ -184,6 +185,68 @@ FCA_HDI int speed_may_exceed(R h, R n, R s_hi, int flat, R thr, R g)
         const R rhs = (tn2 * tn2) * tn2;
         return !(h >= (R)0x1p-20 && s_hi >= (R)0x1p-30 && lhs >= (R)0x1p-100 && lhs < rhs);
     }
+}
+
+/* The same test in base-2 logarithms on the device (float): with lg the
+ * hardware log2, whose absolute error over every positive normal float is at
+ * most 2^-14 (checked on the GPU over all of them, tests/test_gpu_math.py),
+ *   L   = (2/3) lg h + (1/2) lg s_hi - lg n'   bounds log2 of the Manning speed,
+ *   Lcr = (1/2) lg h + (1/2) lg g              bounds log2 of the critical speed,
+ * each within 3.2 * 2^-14 + 2^-15 of the exact value (the lg errors, the
+ * roundings of the fmas at |L| < 512 and of RN(2/3)), and the computed speed
+ * is within 2^-21 of the exact product in log2. So min(L, Lcr) < lthr =
+ * lg(thr) - 2^-10 implies v < thr. Both logs need normal arguments; lthr is
+ * -inf until the wavefront has a thr >= 1e-6 (the speed floor), so nothing
+ * is skipped before then. lgg = (1/2) lg g. */
+FCA_HDI float speed_log_thr(float thr)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (thr >= 1e-6f && thr <= 0x1p100f) ? __builtin_amdgcn_logf(thr) - 0x1p-10f
+                                              : -INFINITY;
+#else
+    (void)thr;
+    return -INFINITY;
+#endif
+}
+FCA_HDI double speed_log_thr(double) { return -INFINITY; }
+
+template <class R>
+FCA_HDI int speed_may_exceed_fast(R h, R n, R s_hi, int flat, R thr, R lthr, R lgg, R g)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (sizeof(R) == 4) {
+        if (flat) {
+            return !(thr >= (R)1e-6);
+        }
+        {
+            const float nn = (n < 1e-12f) ? 1e-12f : (float)n;
+            const float lh = __builtin_amdgcn_logf((float)h);
+            const float lcr = __builtin_fmaf(lh, 0.5f, (float)lgg);
+            const float l = __builtin_fmaf(
+                lh, 2.0f / 3.0f,
+                __builtin_fmaf(__builtin_amdgcn_logf((float)s_hi), 0.5f, -__builtin_amdgcn_logf(nn)));
+            const float lm = (l < lcr) ? l : lcr;
+            return !(lm < (float)lthr && __builtin_fminf((float)h, (float)s_hi) >= 0x1p-126f);
+        }
+    }
+#endif
+    (void)lthr, (void)lgg;
+    return speed_may_exceed<R>(h, n, s_hi, flat, thr, g);
+}
+
+/* (1/2) lg g for speed_may_exceed_fast (device, float). */
+template <class R>
+FCA_HDI R speed_log_half_g(R g)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (sizeof(R) == 4) {
+        /* +inf (no skip by the critical speed) unless g is a normal float */
+        return (g >= (R)0x1p-126 && g <= (R)0x1p127) ? (R)(0.5f * __builtin_amdgcn_logf((float)g))
+                                                    : (R)INFINITY;
+    }
+#endif
+    (void)g;
+    return (R)0;
 }
 
 /* Open-boundary faces of a LAND cell of row q (model.c:166-177). */
@@ -591,8 +654,8 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 s_hi = (sd > s_hi) ? sd : s_hi;
             }
             s_hi = s_hi * ((R)1 + (R)0x1p-20);
-            s.dneed =
-                speed_may_exceed<R>(s.hn3, s.n3, s_hi, ax == (R)0 && ad == (R)0, s.vthr, p.g);
+            s.dneed = speed_may_exceed_fast<R>(s.hn3, s.n3, s_hi, ax == (R)0 && ad == (R)0,
+                                               s.vthr, s.lthr, s.lgg, p.g);
         });
         if (lp.any(&L::dneed)) {
             lp.each([&](L &s, int) {
@@ -611,6 +674,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 }
             });
             lp.wave_max_to(&L::vthr, &L::vmax);
+            lp.each([&](L &s, int) { s.lthr = speed_log_thr(s.vthr); });
         }
     }
 }
@@ -640,6 +704,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         s.colok = s.c >= 0 && s.c < nc;
         s.outcol = s.colok && lane >= FCA_HALO && lane < FCA_WAVE - FCA_HALO;
         s.boff = (uint32_t)s.c * (uint32_t)sizeof(R);
+        s.lgg = speed_log_half_g<R>(a.p.g);
         s.z0 = s.z1 = s.z2 = s.z3 = s.z4 = (R)0;
         s.m0 = s.m1 = s.m2 = s.m3 = s.m4 = FCA_NONE;
         s.h0 = s.h1 = s.h2 = (R)0;
@@ -796,6 +861,7 @@ FCA_HD void walk_acc_init(LP &lp)
         s.dtc = (R)INFINITY;
         s.vmax = (R)0;
         s.vthr = (R)0;
+        s.lthr = (R)-INFINITY;
     });
 }
 

@@ -156,11 +156,12 @@ class Dataset:
 
 
 def ca_targets(base: Grid, scens, R, T, W, row0=0, batch=512, prec="f32", strip_rows=0,
-               timing=None):
+               timing=None, log=None):
     """Runs scens through the batched CA (chunks of `batch`) and writes their
     targets into rows row0.. of T and W (device): hmax / storm depth, LAND
     fraction x (depth / D0)^2. Returns the per-scenario CA results
-    [(flooded, max depth, steps)]."""
+    [(flooded, max depth, steps)]. log, if given, receives one line per
+    chunk."""
     torch = _torch()
     L = nn.lib()
     out = []
@@ -195,6 +196,9 @@ def ca_targets(base: Grid, scens, R, T, W, row0=0, batch=512, prec="f32", strip_
         t_ca += (t1 - t0) + (t3 - t2)
         t_tg += t2 - t1
         del cases
+        if log is not None:
+            log(f"CA chunk {c0 // batch + 1}/{-(-len(scens) // batch)}: {len(chunk)} "
+                f"scenarios in {t3 - t0:.1f}s")
     if timing is not None:
         timing["ca"] = timing.get("ca", 0.0) + t_ca
         timing["target"] = timing.get("target", 0.0) + t_tg
@@ -202,7 +206,7 @@ def ca_targets(base: Grid, scens, R, T, W, row0=0, batch=512, prec="f32", strip_
 
 
 def make_dataset(base: Grid, terrain: Terrain, scens, R=256, batch=512, prec="f32",
-                 strip_rows=0) -> Dataset:
+                 strip_rows=0, log=None) -> Dataset:
     torch = _torch()
     N = len(scens)
     q = R // 4
@@ -211,7 +215,7 @@ def make_dataset(base: Grid, terrain: Terrain, scens, R=256, batch=512, prec="f3
     T = torch.empty((N, R, R), dtype=torch.float32, device="cuda")
     W = torch.empty((N, R, R), dtype=torch.float32, device="cuda")
     timing = {}
-    ca = ca_targets(base, scens, R, T, W, 0, batch, prec, strip_rows, timing)
+    ca = ca_targets(base, scens, R, T, W, 0, batch, prec, strip_rows, timing, log)
     t0 = time.perf_counter()
     smp, S, _ = terrain.upload(scens, R)
     Lf = torch.empty((min(N, 512), R, R), dtype=torch.float32, device="cuda")
@@ -374,7 +378,7 @@ def select(sc: Screen, frac_area=0.02, frac_unc=0.02):
 
 
 def judged_error(net: nn.Net, terrain: Terrain, base: Grid, scens, R=256, batch=512,
-                 strip_rows=0, timing=None):
+                 strip_rows=0, timing=None, log=None):
     """CA reruns of scens: returns (per-scenario CA results, per-scenario
     (sum Lf*(pred - hmax)^2 in m^2, sum Lf) of the surrogate's prediction
     against the CA's target, per-scenario target moments as row_moments in
@@ -384,7 +388,7 @@ def judged_error(net: nn.Net, terrain: Terrain, base: Grid, scens, R=256, batch=
     q = R // 4
     T = torch.empty((N, R, R), dtype=torch.float32, device="cuda")
     W = torch.empty((N, R, R), dtype=torch.float32, device="cuda")
-    ca = ca_targets(base, scens, R, T, W, 0, batch, "f32", strip_rows, timing)
+    ca = ca_targets(base, scens, R, T, W, 0, batch, "f32", strip_rows, timing, log)
     B = 64
     X = torch.empty((B, q, q, 64), dtype=torch.uint16, device="cuda")
     Lf = torch.empty((B, R, R), dtype=torch.float32, device="cuda")
@@ -464,7 +468,8 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     res["t_terrain_s"] = time.perf_counter() - t0
     scens = E.draw(pc.n_train, pc.seed, base.shape, side=pc.side, dur=pc.dur)
     t0 = time.perf_counter()
-    data = make_dataset(base, terrain, scens, pc.R, pc.ca_batch)
+    data = make_dataset(base, terrain, scens, pc.R, pc.ca_batch,
+                        log=lambda m: log(f"surrogate: dataset {m}"))
     res["t_dataset_s"] = time.perf_counter() - t0
     res["dataset_ca_s"] = data.t_ca
     log(f"surrogate: dataset of {pc.n_train} CA runs in {res['t_dataset_s']:.1f}s "
@@ -517,7 +522,8 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     timing = {}
     t0 = time.perf_counter()
     ca, se, mo = judged_error(net, terrain, base, [cands[i] for i in judged], pc.R,
-                              pc.ca_batch, timing=timing)
+                              pc.ca_batch, timing=timing,
+                              log=lambda m: log(f"surrogate: rerun {m}"))
     res["t_rerun_s"] = time.perf_counter() - t0
     na = len(top_a)
     res["judged"] = int(len(judged))

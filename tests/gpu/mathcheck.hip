@@ -58,6 +58,23 @@ __global__ void k_eval(int kind, uint32_t first, uint32_t count, McArgs a, float
         break;
     }
     case 12: y = div_step_cr(x, a.d, a.rd); break;
+#if defined(__HIP_DEVICE_COMPILE__)
+    case 13: /* absolute error of the hardware log2 against log2 in double */
+        y = (float)__builtin_fabs((double)__builtin_amdgcn_logf(x) - log2((double)x));
+        break;
+#endif
+    case 14: { /* stage D bound in logarithms: as case 11 with speed_may_exceed_fast */
+        const float h = x, n = a.e, ax = a.f;
+        const float s_hi = (ax * a.rd) * (1.0f + 0x1p-20f);
+        const float smax = div_c(ax, a.d, a.rd, a.dok);
+        const float v = cell_speed<float>(h, n, smax, p);
+        const float thr = f32_step(v, a.k);
+        const int need = speed_may_exceed_fast<float>(h, n, s_hi, ax == 0.0f, thr,
+                                                      speed_log_thr(thr),
+                                                      speed_log_half_g<float>(a.g), a.g);
+        y = (!need && v > thr) ? 1.0f : (need ? 0.5f : 0.0f);
+        break;
+    }
     default: y = sqrt_rough(x); break;
     }
     out[i] = y;

@@ -232,3 +232,35 @@ def test_step_constant_division_is_correctly_rounded(den):
         x = bits(first, n).view(np.float32)
         idx = same(got, x / d)
         assert idx.size == 0, f"x/{d} mismatch at {hex(first + int(idx[0]))}"
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_hardware_log2_error_bound_on_every_normal_float():
+    """The stage D filter in logarithms (speed_may_exceed_fast) assumes the
+    hardware log2 is within 2^-14 of log2 on every positive normal float."""
+    lib = _lib()
+    print("exhaustive log2", flush=True)
+    worst = 0.0
+    for first in range(0x00800000, TOP, CHUNK):
+        n = min(CHUNK, TOP - first)
+        worst = max(worst, float(dev(lib, 13, first, n).max()))
+    print(f"max |lg - log2| = {worst:.3e}", flush=True)
+    assert worst <= 2.0 ** -14
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("n,ax", [(0.02, 0.5), (0.15, 3e-4), (0.035, 0.0), (1e-13, 2.0),
+                                  (0.06, 40.0)])
+@pytest.mark.parametrize("k", [-1, -4, -4000, 1])
+def test_stage_d_log_bound_never_skips_a_faster_cell(n, ax, k):
+    """As the sixth-power bound: for every depth in [1e-3, 1e3] and thr one or
+    more ulps below (k < 0) or above the cell's exact speed, the log2 bound
+    never skips a cell that is faster than thr."""
+    lib = _lib()
+    for first, cnt in _range(1e-3, 1e3):
+        got = dev(lib, 14, first, cnt, e=n, f=ax, k=k)
+        assert not np.any(got == 1.0), f"skipped a faster cell (n={n}, ax={ax}, k={k})"
+        if k < 0 and ax > 0:
+            assert np.all(got == 0.5)
