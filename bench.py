@@ -16,7 +16,12 @@ Printed: one JSON line with value = LAND cell-updates/s (the reference's MCUPS
 definition x 1e6, src/common/log.c:133-135), the HBM roofline of the step
 kernel (algorithmic bytes per launch / average launch time vs 8 TB/s), and
 the reference serial build's throughput on the host cores (a bounded sample
-of the 512^2 case) as the CPU baseline.
+of the 512^2 case) as the CPU baseline. Nested in the same line, each with
+its own workload: the 4096^2 grid in FP32 and FP64, the sparse start on 8192^2 (quiet-tile skipping against the
+dense walk), a 96-scenario ensemble (batched against one session per
+scenario) and the surrogate-screened ensemble (2048 CA runs of training data
+and 20000 candidates by default; --sur-train 4096 --sur-cand 100000 is the
+full case).
 """
 from __future__ import annotations
 
@@ -225,6 +230,42 @@ def sparse_bench(prec: str, size: int = 8192, hours: float = 8.0):
                         f"peaking at 2 h over {hours:g} h, no rain, FP32, K=4"}
 
 
+def grid4096_bench(steps: int = 500, warmup: int = 300):
+    """The plain method on the 4096^2 grid of the same generator in FP32 and
+    in FP64 (the double-precision mode of the reference's comparison), timed
+    as the main line: warmup steps from a dry start under 50 mm/h, then steps
+    timed with HIP events on the session's stream."""
+    out = {}
+    for prec in ("f32", "f64"):
+        a = A.abi(prec)
+        g = synth.make_grid(4096, 4096, seed=20261203, dtype=a.dtype)
+        cfg = A.default_cfg(a)
+        A.set_dx(a, cfg, g.dx)
+        cfg.hyeto = A.FCA_HYETO_UNIFORM
+        cfg.T_sim = 1e9
+        acc = a.FcaAcc()
+        acc.force_day = -1
+        timer = HipTimer()
+        with Sim(g, cfg, acc, prec=prec) as sim:
+            stream = C.c_void_p(sim.stream)
+            sim.advance(warmup)
+            sim.sync()
+            timer.device_sync()
+            timer.start(stream)
+            sim.advance(steps)
+            ms = timer.stop(stream)
+            res = sim.read()
+        bps = bytes_per_step(g.n_land, g.z.size, prec)
+        out[prec] = {"ms_per_step": ms / steps,
+                     "land_cell_updates_per_s": g.n_land * steps / (ms * 1e-3),
+                     "algorithmic_gb_per_s": bps / (ms / steps * 1e-3) / 1e9,
+                     "steps": steps, "warmup": warmup, "steps_taken": res.step}
+        print(f"[bench] 4096^2 {prec}: {ms / steps:.4f} ms/step", file=sys.stderr, flush=True)
+    out["workload"] = ("4096^2 at 90 m, same generator, 50 mm/h uniform rain from a dry start, "
+                       "K=4, U=1, fused kernel; FP32 and FP64")
+    return out
+
+
 def forward_flops(cfg) -> int:
     """Multiply-adds x 2 of one sample's forward pass (every GEMM's M*N*K)."""
     from floodca_amd import nn
@@ -298,13 +339,17 @@ def main() -> int:
                     help="of which this many are also run one session each")
     ap.add_argument("--sparse-size", type=int, default=8192,
                     help="grid side of the sparse-start measurement (0 = skip)")
-    ap.add_argument("--surrogate", choices=("off", "on"), default="off",
+    ap.add_argument("--surrogate", choices=("off", "on"), default="on",
                     help="run the surrogate-screened ensemble after the main line")
-    ap.add_argument("--sur-train", type=int, default=4096, help="CA runs of its dataset")
-    ap.add_argument("--sur-cand", type=int, default=100000, help="candidates it screens")
+    ap.add_argument("--sur-train", type=int, default=2048,
+                    help="CA runs of its dataset (the full case: 4096)")
+    ap.add_argument("--sur-cand", type=int, default=20000,
+                    help="candidates it screens (the full case: 100000)")
     ap.add_argument("--sur-epochs", type=int, default=30)
-    ap.add_argument("--sur-alone", type=int, default=64,
+    ap.add_argument("--sur-alone", type=int, default=48,
                     help="candidates timed one session each (the all-CA baseline)")
+    ap.add_argument("--no-4096", dest="grid4096", action="store_false",
+                    help="skip the 4096^2 FP32/FP64 measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -392,6 +437,8 @@ def main() -> int:
             "bytes_rule": "per LAND cell z,n,cr,h,hmax + mask read, h written; 1 B mask per other cell",
         },
     }
+    if args.grid4096 and args.size != 4096 and prec == "f32" and args.neigh == 4:
+        line["grid_4096"] = grid4096_bench()
     if args.sparse_size > 0 and prec == "f32" and args.neigh == 4:
         line["sparse_start"] = sparse_bench(prec, args.sparse_size)
     if args.ens_scenarios > 0 and prec == "f32" and args.neigh == 4:

@@ -42,6 +42,7 @@ typedef struct FcaDev {
     int skip;
     double *red;            /* scratch of the volume / ensemble reductions */
     int nred;
+    void *arena;            /* one allocation holding every buffer above */
 } FcaDev;
 
 /* Device table of n grids for the batched fused step (NULL on failure); free

@@ -111,16 +111,37 @@ static int launch_sim_step(void *ctx, long long k)
 
 static void dev_free(FcaDev *d)
 {
-    void *ptrs[] = {d->z, d->n, d->C, d->Rday, d->cr, d->h[0], d->h[1], d->hmax, d->tmax,
-                    d->mask, d->qx, d->qy, d->qd1, d->qd2, d->qo, d->s, d->sv, d->ctrl,
-                    d->cand, d->slots, d->rec, d->red, d->tflags, d->tnorain};
-    size_t i;
-    for (i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) {
-        if (ptrs[i]) {
-            hipFree(ptrs[i]);
-        }
+    if (d->arena) {
+        hipFree(d->arena);
     }
     memset(d, 0, sizeof(*d));
+}
+
+/* Carves a session's device buffers out of one allocation: a batch opens
+ * hundreds of sessions, and one hipMalloc/hipFree each instead of one per
+ * buffer takes most of the cost out of opening and closing them. Every
+ * buffer starts on a 256-byte boundary. */
+typedef struct ArenaReq {
+    void **ptr;
+    size_t bytes;
+} ArenaReq;
+
+static int dev_arena(FcaDev *d, const ArenaReq *req, int n)
+{
+    size_t off = 0;
+    int i;
+    char *base;
+    for (i = 0; i < n; i++) {
+        off += (req[i].bytes + 255) & ~(size_t)255;
+    }
+    HIP_OR(hipMalloc(&d->arena, off ? off : 256), -2);
+    base = (char *)d->arena;
+    off = 0;
+    for (i = 0; i < n; i++) {
+        *req[i].ptr = base + off;
+        off += (req[i].bytes + 255) & ~(size_t)255;
+    }
+    return 0;
 }
 
 void fca_hip_sim_close(FcaHipSim *s)
@@ -266,40 +287,52 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
         HIP_OR(hipStreamCreate(&s->st), -2);
         s->own_stream = 1;
     }
-    HIP_OR(hipMalloc((void **)&d->z, B), -2);
-    HIP_OR(hipMalloc((void **)&d->n, B), -2);
-    HIP_OR(hipMalloc((void **)&d->C, B), -2);
-    HIP_OR(hipMalloc((void **)&d->Rday, B), -2);
-    HIP_OR(hipMalloc((void **)&d->cr, B), -2);
-    HIP_OR(hipMalloc((void **)&d->h[0], B), -2);
-    HIP_OR(hipMalloc((void **)&d->h[1], B), -2);
-    HIP_OR(hipMalloc((void **)&d->hmax, B), -2);
-    HIP_OR(hipMalloc((void **)&d->tmax, B), -2);
-    HIP_OR(hipMalloc((void **)&d->mask, N), -2);
-    HIP_OR(hipMalloc((void **)&d->ctrl, 2 * sizeof(FcaDevCtrl)), -2);
-    HIP_OR(hipMalloc((void **)&d->cand, 3 * sizeof(double)), -2);
     d->nslots = d->kernel == FCA_HIP_FUSED ? d->nblocks : 1;
-    HIP_OR(hipMalloc((void **)&d->slots, (size_t)d->nslots * sizeof(FcaAccSlot)), -2);
-    HIP_OR(hipMalloc((void **)&d->rec, (size_t)(d->rec_cap > 0 ? d->rec_cap : 1) *
-                                           sizeof(FcaHipStepRec)), -2);
     d->nred = NRED * 5;
-    HIP_OR(hipMalloc((void **)&d->red, (size_t)d->nred * sizeof(double)), -2);
+    {
+        ArenaReq req[24];
+        int nq = 0;
+#define FCA_REQ(p_, bytes_) (req[nq].ptr = (void **)&(p_), req[nq].bytes = (bytes_), nq++)
+        FCA_REQ(d->z, B);
+        FCA_REQ(d->n, B);
+        FCA_REQ(d->C, B);
+        FCA_REQ(d->Rday, B);
+        FCA_REQ(d->cr, B);
+        FCA_REQ(d->h[0], B);
+        FCA_REQ(d->h[1], B);
+        FCA_REQ(d->hmax, B);
+        FCA_REQ(d->tmax, B);
+        FCA_REQ(d->mask, N);
+        FCA_REQ(d->ctrl, 2 * sizeof(FcaDevCtrl));
+        FCA_REQ(d->cand, 3 * sizeof(double));
+        FCA_REQ(d->slots, (size_t)d->nslots * sizeof(FcaAccSlot));
+        FCA_REQ(d->rec, (size_t)(d->rec_cap > 0 ? d->rec_cap : 1) * sizeof(FcaHipStepRec));
+        FCA_REQ(d->red, (size_t)d->nred * sizeof(double));
+        if (d->kernel == FCA_HIP_FUSED) {
+            FCA_REQ(d->tflags, (size_t)2 * d->ntiles);
+            FCA_REQ(d->tnorain, (size_t)d->ntiles);
+        }
+        if (d->kernel == FCA_HIP_SPLIT) {
+            size_t nqx = (size_t)d->nr * (size_t)(d->nc - 1), nqy = (size_t)(d->nr - 1) * d->nc;
+            size_t nqd = (size_t)(d->nr - 1) * (size_t)(d->nc - 1);
+            FCA_REQ(d->qx, (nqx ? nqx : 1) * sizeof(real));
+            FCA_REQ(d->qy, (nqy ? nqy : 1) * sizeof(real));
+            FCA_REQ(d->qd1, (nqd ? nqd : 1) * sizeof(real));
+            FCA_REQ(d->qd2, (nqd ? nqd : 1) * sizeof(real));
+            FCA_REQ(d->qo, 4 * B);
+            FCA_REQ(d->s, B);
+            FCA_REQ(d->sv, 256);
+        }
+#undef FCA_REQ
+        if (dev_arena(d, req, nq) != 0) {
+            return -2;
+        }
+    }
     if (d->kernel == FCA_HIP_FUSED) {
         s->skip_wanted = !(o && (o->flags & FCA_HIP_NO_SKIP));
         d->skip = 0;
-        HIP_OR(hipMalloc((void **)&d->tflags, (size_t)2 * d->ntiles), -2);
-        HIP_OR(hipMalloc((void **)&d->tnorain, (size_t)d->ntiles), -2);
     }
     if (d->kernel == FCA_HIP_SPLIT) {
-        size_t nqx = (size_t)d->nr * (size_t)(d->nc - 1), nqy = (size_t)(d->nr - 1) * d->nc;
-        size_t nqd = (size_t)(d->nr - 1) * (size_t)(d->nc - 1);
-        HIP_OR(hipMalloc((void **)&d->qx, (nqx ? nqx : 1) * sizeof(real)), -2);
-        HIP_OR(hipMalloc((void **)&d->qy, (nqy ? nqy : 1) * sizeof(real)), -2);
-        HIP_OR(hipMalloc((void **)&d->qd1, (nqd ? nqd : 1) * sizeof(real)), -2);
-        HIP_OR(hipMalloc((void **)&d->qd2, (nqd ? nqd : 1) * sizeof(real)), -2);
-        HIP_OR(hipMalloc((void **)&d->qo, 4 * B), -2);
-        HIP_OR(hipMalloc((void **)&d->s, B), -2);
-        HIP_OR(hipMalloc((void **)&d->sv, 256), -2);
         HIP_OR(hipMemset(d->qo, 0, 4 * B), -2);
     }
     /* small read-backs go to ordinary memory: pinning costs more than it
@@ -1187,8 +1220,7 @@ FcaHipBatch *fca_hip_batch_open(int n, const FcaGrid *const *g, const FcaRunCfg 
             fail(-2, "controller pool copy failed");
             goto fail_out;
         }
-        hipFree(d->ctrl);
-        d->ctrl = &b->ctrl_pool[2 * i];
+        d->ctrl = &b->ctrl_pool[2 * i]; /* the session's own pair stays unused in its arena */
         devs[i] = d;
     }
     b->tab = fca_k_batch_table((const FcaDev *const *)devs, n);
