@@ -21,6 +21,7 @@ FCA_HYETO_TRIANGLE, FCA_HYETO_UNIFORM = 0, 1
 FCA_MAX_FORCE_DAYS = 64
 FCA_HIP_FUSED, FCA_HIP_SPLIT = 0, 1
 FCA_HIP_NO_LAND_FORM, FCA_HIP_NO_SKIP = 1, 2
+FCA_HIP_MAX_CLASSES = 16
 
 
 def _make(real):
@@ -87,9 +88,13 @@ def _make(real):
         _fields_ = [("t", C.c_double), ("F", C.c_double), ("dt_base", real), ("dt", real),
                     ("step", C.c_int), ("hit_event", C.c_int)]
 
+    class FcaHipCrop(C.Structure):
+        _fields_ = [("r0", C.c_int), ("c0", C.c_int), ("nrows", C.c_int), ("ncols", C.c_int),
+                    ("rday", real), ("n_class", real * FCA_HIP_MAX_CLASSES)]
+
     return dict(FcaPhys=FcaPhys, FcaRunCfg=FcaRunCfg, FcaComm=FcaComm, FcaGrid=FcaGrid,
                 FcaAcc=FcaAcc, FcaTimers=FcaTimers, FcaHipOpts=FcaHipOpts,
-                FcaHipStepRec=FcaHipStepRec)
+                FcaHipStepRec=FcaHipStepRec, FcaHipCrop=FcaHipCrop)
 
 
 @dataclass(frozen=True)

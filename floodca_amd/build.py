@@ -37,6 +37,23 @@ EMU_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno
 PRECS = {"f32": ([], "libfloodca_hip.so"), "f64": (["-DFCA_DOUBLE"], "libfloodca_hip_f64.so")}
 
 
+def load_hip_library(path: str):
+    """ctypes handle of one of this package's HIP libraries, with one HIP
+    runtime per process: PyTorch's ROCm build carries its own copy of
+    libamdhip64 under a file name its libraries ask for by path, so a process
+    that loads ours (linked against /opt/rocm's) first and imports torch
+    later holds two runtimes, and torch's may then find no device. Importing
+    torch first, where it is installed, makes our libraries bind to the copy
+    torch loaded (the same soname)."""
+    import ctypes
+
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    return ctypes.CDLL(path)
+
+
 def lib_path(prec: str = "f32") -> str:
     """The in-tree backend library; FCA_HIP_LIB (float) / FCA_HIP_LIB_F64 override it."""
     env = os.environ.get("FCA_HIP_LIB" if prec == "f32" else "FCA_HIP_LIB_F64")

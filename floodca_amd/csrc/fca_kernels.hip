@@ -664,6 +664,29 @@ __global__ void __launch_bounds__(FCA_BLOCK) k_cr(const R *C, const R *Rday, R *
     }
 }
 
+/* A crop scenario's fields from the resident base grid: z and mask copied
+ * from the window at (r0, c0), n from the class of each cell, C = 1,
+ * Rday = rday, and a dry start (h = hmax = tmax = 0 in both depth buffers).
+ * One thread per cell of the crop, rows of ncols. */
+__global__ void __launch_bounds__(FCA_BLOCK)
+    k_crop(const R *bz, const uint8_t *bmask, const uint8_t *bcls, int bnc, int r0, int c0,
+           int nr, int nc, FcaHipCrop crop, R *z, uint8_t *mask, R *n, R *C, R *Rday, R *h0,
+           R *h1, R *hmax, R *tmax)
+{
+    const size_t N = (size_t)nr * nc;
+    for (size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * FCA_BLOCK) {
+        const int r = (int)(i / (size_t)nc), c = (int)(i - (size_t)r * nc);
+        const size_t b = (size_t)(r0 + r) * (size_t)bnc + (size_t)(c0 + c);
+        z[i] = bz[b];
+        mask[i] = bmask[b];
+        n[i] = crop.n_class[bcls[b] & (FCA_HIP_MAX_CLASSES - 1)];
+        C[i] = (R)1;
+        Rday[i] = crop.rday;
+        h0[i] = h1[i] = hmax[i] = tmax[i] = (R)0;
+    }
+}
+
 /* Fixed-partition partial sums, so the total is the same on every run. */
 __global__ void __launch_bounds__(FCA_BLOCK)
     k_volume(const R *h, const uint8_t *mask, size_t N, R A, double *red)
@@ -931,6 +954,17 @@ extern "C" int fca_k_tile_norain(const FcaDev *d, hipStream_t st)
     }
     hipLaunchKernelGGL(k_tile_norain, dim3(d->ntiles), dim3(64), 0, st, d->cr, d->mask, d->nr,
                        d->nc, d->strip_rows, d->ngroups, d->tnorain);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fca_k_crop(const FcaDev *d, const real *bz, const uint8_t *bmask,
+                          const uint8_t *bcls, int bnc, const FcaHipCrop *crop, hipStream_t st)
+{
+    const size_t N = (size_t)d->nr * d->nc;
+    unsigned g = blocks_for(N);
+    hipLaunchKernelGGL(k_crop, dim3(g < 65536 ? g : 65536), dim3(FCA_BLOCK), 0, st, bz, bmask,
+                       bcls, bnc, crop->r0, crop->c0, d->nr, d->nc, *crop, d->z, d->mask, d->n,
+                       d->C, d->Rday, d->h[0], d->h[1], d->hmax, d->tmax);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

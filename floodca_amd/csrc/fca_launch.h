@@ -64,6 +64,10 @@ int fca_k_step(const FcaDev *d, long long k, hipStream_t st);
 int fca_k_tile_norain(const FcaDev *d, hipStream_t st);
 /* cr = C*Rday on every cell. */
 int fca_k_cr(const FcaDev *d, hipStream_t st);
+/* Fills d's z, mask, n, C, Rday and a dry state from a window of a resident
+ * base grid (bz, bmask, bcls with bnc columns), as crop describes. */
+int fca_k_crop(const FcaDev *d, const real *bz, const uint8_t *bmask, const uint8_t *bcls,
+               int bnc, const FcaHipCrop *crop, hipStream_t st);
 /* Sets candidate slot `slot` to +inf. */
 int fca_k_cand_reset(const FcaDev *d, int slot, hipStream_t st);
 /* Per-block partial LAND volumes sum(h*A) of h[hbuf] into d->red[0..nblk). Returns nblk. */

@@ -360,13 +360,13 @@ static int sim_upload_static(FcaHipSim *s, const FcaGrid *g)
     return 0;
 }
 
-/* Uploads the per-run state (h, hmax, tmax, Rday), resets the controller
- * from acc and computes the first time-step candidate. */
+static int sim_start(FcaHipSim *s, const FcaAcc *acc);
+
+/* Uploads the per-run state (h, hmax, tmax, Rday), then sim_start. */
 static int sim_upload_state(FcaHipSim *s, const FcaGrid *g, const FcaAcc *acc)
 {
     FcaDev *d = &s->d;
     const size_t B = s->N * sizeof(real);
-    FcaDevCtrl c;
     double t0 = now_s();
     HIP_OR(hipMemcpy(d->h[0], g->h, B, hipMemcpyHostToDevice), -2);
     HIP_OR(hipMemcpy(d->h[1], d->h[0], B, hipMemcpyDeviceToDevice), -2);
@@ -374,6 +374,16 @@ static int sim_upload_state(FcaHipSim *s, const FcaGrid *g, const FcaAcc *acc)
     HIP_OR(hipMemcpy(d->tmax, g->tmax, B, hipMemcpyHostToDevice), -2);
     HIP_OR(hipMemcpy(d->Rday, g->Rday, B, hipMemcpyHostToDevice), -2);
     s->t_h2d += now_s() - t0;
+    return sim_start(s, acc);
+}
+
+/* With the fields on the device: resets the controller from acc, clears the
+ * slots and tile flags, computes cr = C*Rday, the rain-free tiles and the
+ * first time-step candidate. */
+static int sim_start(FcaHipSim *s, const FcaAcc *acc)
+{
+    FcaDev *d = &s->d;
+    FcaDevCtrl c;
     memset(&c, 0, sizeof(c));
     c.t = acc->t;
     c.step = acc->step;
@@ -474,6 +484,168 @@ static FcaHipSim *sim_create(const FcaGrid *g, const FcaRunCfg *cfg, const FcaAc
     s->faces_per_step = count_faces(g, &cfg->phys);
     if (sim_alloc(s, g, cfg, opts, shared) != 0 || sim_upload_static(s, g) != 0 ||
         sim_upload_state(s, g, acc) != 0) {
+        char keep[512];
+        memcpy(keep, g_err, sizeof(keep));
+        fca_hip_sim_close(s);
+        memcpy(g_err, keep, sizeof(keep));
+        return NULL;
+    }
+    return s;
+}
+
+/* ---- crop scenarios built on the device ---------------------------------- */
+
+struct FcaHipBase {
+    int nr, nc;
+    real *z;
+    uint8_t *mask, *cls;
+    uint8_t *h_mask;    /* host copy, for the face counts of a window */
+    unsigned classes;   /* bit k: class k occurs somewhere in the base */
+};
+
+FcaHipBase *fca_hip_base_open(const FcaGrid *g, const uint8_t *cls)
+{
+    FcaHipBase *b;
+    size_t N, i;
+    g_err[0] = 0;
+    if (!g || !cls || !g->z || !g->mask || g->nrows <= 0 || g->ncols <= 0) {
+        fail(-1, "bad arguments");
+        return NULL;
+    }
+    N = (size_t)g->nrows * (size_t)g->ncols;
+    b = (FcaHipBase *)calloc(1, sizeof(*b));
+    if (!b) {
+        fail(-2, "out of host memory");
+        return NULL;
+    }
+    b->nr = g->nrows;
+    b->nc = g->ncols;
+    for (i = 0; i < N; i++) {
+        if (cls[i] >= FCA_HIP_MAX_CLASSES) {
+            fail(-1, "class %d at cell %zu is not below %d", cls[i], i, FCA_HIP_MAX_CLASSES);
+            free(b);
+            return NULL;
+        }
+        b->classes |= 1u << cls[i];
+    }
+    b->h_mask = (uint8_t *)malloc(N);
+    if (!b->h_mask || hipMalloc((void **)&b->z, N * sizeof(real)) != hipSuccess ||
+        hipMalloc((void **)&b->mask, N) != hipSuccess ||
+        hipMalloc((void **)&b->cls, N) != hipSuccess ||
+        hipMemcpy(b->z, g->z, N * sizeof(real), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->mask, g->mask, N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->cls, cls, N, hipMemcpyHostToDevice) != hipSuccess) {
+        fail(-2, "base grid upload failed");
+        fca_hip_base_close(b);
+        return NULL;
+    }
+    memcpy(b->h_mask, g->mask, N);
+    return b;
+}
+
+void fca_hip_base_close(FcaHipBase *b)
+{
+    if (!b) {
+        return;
+    }
+    if (b->z) {
+        hipFree(b->z);
+    }
+    if (b->mask) {
+        hipFree(b->mask);
+    }
+    if (b->cls) {
+        hipFree(b->cls);
+    }
+    free(b->h_mask);
+    free(b);
+}
+
+/* count_faces for a window of the base (its mask read from the host copy). */
+static long count_faces_window(const FcaHipBase *b, const FcaHipCrop *w, const FcaPhys *p)
+{
+    long nr = w->nrows, nc = w->ncols, f, r, c;
+    f = nr * (nc - 1) + (nr - 1) * nc + (p->K == 8 ? 2 * (nr - 1) * (nc - 1) : 0);
+    if (p->boundary == FCA_BND_OPEN) {
+        /* open faces of LAND cells on the window's edges, as count_faces */
+        for (r = 0; r < nr; r++) {
+            const uint8_t *row = b->h_mask + (size_t)(w->r0 + r) * (size_t)b->nc + w->c0;
+            const long er = (r == 0) + (r == nr - 1);
+            if (er) {
+                for (c = 0; c < nc; c++) {
+                    if (row[c] == FCA_LAND) {
+                        f += er + (c == 0) + (c == nc - 1);
+                    }
+                }
+            } else {
+                if (row[0] == FCA_LAND) {
+                    f += 1 + (nc == 1);
+                }
+                if (nc > 1 && row[nc - 1] == FCA_LAND) {
+                    f += 1;
+                }
+            }
+        }
+    }
+    return f;
+}
+
+static FcaHipSim *sim_create_crop(const FcaHipBase *b, const FcaHipCrop *w,
+                                  const FcaRunCfg *cfg, const FcaHipOpts *opts,
+                                  hipStream_t shared)
+{
+    FcaHipSim *s;
+    FcaGrid gh;
+    FcaAcc acc;
+    unsigned k;
+    g_err[0] = 0;
+    memset(&gh, 0, sizeof(gh));
+    gh.nrows = w->nrows;
+    gh.ncols = w->ncols;
+    if (check_cfg(&gh, cfg, opts) != 0) {
+        return NULL;
+    }
+    if (w->r0 < 0 || w->c0 < 0 || w->r0 + w->nrows > b->nr || w->c0 + w->ncols > b->nc) {
+        fail(-1, "crop window outside the base grid");
+        return NULL;
+    }
+    s = (FcaHipSim *)calloc(1, sizeof(*s));
+    if (!s) {
+        fail(-2, "out of host memory");
+        return NULL;
+    }
+    s->cfg = (FcaRunCfg *)malloc(sizeof(FcaRunCfg));
+    if (!s->cfg) {
+        free(s);
+        fail(-2, "out of host memory");
+        return NULL;
+    }
+    memcpy(s->cfg, cfg, sizeof(FcaRunCfg));
+    s->faces_per_step = count_faces_window(b, w, &cfg->phys);
+    /* dry start: fca_acc_init of h = 0 (model.c:102-109) */
+    memset(&acc, 0, sizeof(acc));
+    acc.force_day = -1;
+    if (sim_alloc(s, &gh, cfg, opts, shared) != 0 ||
+        fca_k_crop(&s->d, b->z, b->mask, b->cls, b->nc, w, s->st) != 0) {
+        char keep[512];
+        memcpy(keep, g_err, sizeof(keep));
+        fca_hip_sim_close(s);
+        memcpy(g_err, keep, sizeof(keep));
+        if (!g_err[0]) {
+            fail(-2, "crop launch failed");
+        }
+        return NULL;
+    }
+    /* the roughness of every class that occurs in the base, hence of every
+     * LAND cell of the window, is in the LAND form's range (fca_land_n_ok) */
+    s->n_ok = 1;
+    for (k = 0; k < FCA_HIP_MAX_CLASSES; k++) {
+        if ((b->classes >> k) & 1u) {
+            s->n_ok = s->n_ok && w->n_class[k] >= (real)1e-12 && w->n_class[k] <= (real)0x1p100;
+        }
+    }
+    set_div_flags(&s->d, s->n_ok, s->no_land_form);
+    if (sim_start(s, &acc) != 0) {
         char keep[512];
         memcpy(keep, g_err, sizeof(keep));
         fca_hip_sim_close(s);
@@ -1148,11 +1320,108 @@ void fca_hip_batch_close(FcaHipBatch *b)
     free(b);
 }
 
+/* An empty batch of n scenarios with step kernel K (NULL on failure). */
+static FcaHipBatch *batch_new(int n, int K, const FcaHipOpts *opts)
+{
+    FcaHipBatch *b = (FcaHipBatch *)calloc(1, sizeof(*b));
+    if (!b) {
+        fail(-2, "out of host memory");
+        return NULL;
+    }
+    b->n = n;
+    b->K = K;
+    b->batch = (opts && opts->graph_batch > 0) ? opts->graph_batch : 30;
+    b->sims = (FcaHipSim **)calloc((size_t)n, sizeof(FcaHipSim *));
+    b->active = (char *)calloc((size_t)n, 1);
+    b->last_step = (int *)calloc((size_t)n, sizeof(int));
+    b->k_done = (long long *)calloc((size_t)n, sizeof(long long));
+    if (!b->sims || !b->active || !b->last_step || !b->k_done) {
+        fail(-2, "out of host memory");
+        fca_hip_batch_close(b);
+        return NULL;
+    }
+    if (hipStreamCreate(&b->st) != hipSuccess) {
+        fail(-2, "hipStreamCreate failed");
+        fca_hip_batch_close(b);
+        return NULL;
+    }
+    return b;
+}
+
+/* Scenario i's session is in place: count its workgroups. */
+static void batch_add(FcaHipBatch *b, int i, int step0)
+{
+    b->cap_wg += b->sims[i]->d.nblocks;
+    b->active[i] = 1;
+    b->last_step[i] = step0;
+}
+
+/* With every session open: one pool for every controller record (a single
+ * copy reads them all), the scenario table and the workgroup list. Closes
+ * the batch and returns NULL on failure. */
+static FcaHipBatch *batch_finish(FcaHipBatch *b)
+{
+    FcaDev **devs = (FcaDev **)calloc((size_t)b->n, sizeof(FcaDev *));
+    int i;
+    if (!devs) {
+        fail(-2, "out of host memory");
+        goto fail_out;
+    }
+    if (hipMalloc((void **)&b->ctrl_pool, (size_t)b->n * 2 * sizeof(FcaDevCtrl)) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_ctrl, (size_t)b->n * 2 * sizeof(FcaDevCtrl),
+                      hipHostMallocDefault) != hipSuccess) {
+        fail(-2, "controller pool allocation failed");
+        goto fail_out;
+    }
+    for (i = 0; i < b->n; i++) {
+        FcaDev *d = &b->sims[i]->d;
+        if (hipMemcpyAsync(&b->ctrl_pool[2 * i], d->ctrl, 2 * sizeof(FcaDevCtrl),
+                           hipMemcpyDeviceToDevice, b->st) != hipSuccess) {
+            fail(-2, "controller pool copy failed");
+            goto fail_out;
+        }
+        d->ctrl = &b->ctrl_pool[2 * i]; /* the session's own pair stays unused in its arena */
+        devs[i] = d;
+    }
+    if (hipStreamSynchronize(b->st) != hipSuccess) {
+        fail(-2, "controller pool copy failed");
+        goto fail_out;
+    }
+    b->tab = fca_k_batch_table((const FcaDev *const *)devs, b->n);
+    if (!b->tab) {
+        fail(-2, "scenario table upload failed");
+        goto fail_out;
+    }
+    if (hipMalloc((void **)&b->wgs_dev, (size_t)b->cap_wg * 2 * sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void **)&b->wgs_host, (size_t)b->cap_wg * 2 * sizeof(int),
+                      hipHostMallocDefault) != hipSuccess) {
+        fail(-2, "workgroup list allocation failed");
+        goto fail_out;
+    }
+    if (batch_list(b) != 0) {
+        goto fail_out;
+    }
+    free(devs);
+    return b;
+fail_out:
+    free(devs);
+    fca_hip_batch_close(b);
+    return NULL;
+}
+
+/* Prefixes the current error with the scenario index. */
+static void scenario_error(int i)
+{
+    char keep[sizeof(g_err) + 32];
+    snprintf(keep, sizeof(keep), "scenario %d: %s", i, g_err);
+    memcpy(g_err, keep, sizeof(g_err) - 1);
+    g_err[sizeof(g_err) - 1] = 0;
+}
+
 FcaHipBatch *fca_hip_batch_open(int n, const FcaGrid *const *g, const FcaRunCfg *const *cfg,
                                 const FcaAcc *acc, const FcaHipOpts *opts)
 {
     FcaHipBatch *b;
-    FcaDev **devs = NULL;
     int i;
     g_err[0] = 0;
     if (n <= 0 || !g || !cfg || !acc) {
@@ -1173,76 +1442,60 @@ FcaHipBatch *fca_hip_batch_open(int n, const FcaGrid *const *g, const FcaRunCfg 
             return NULL;
         }
     }
-    b = (FcaHipBatch *)calloc(1, sizeof(*b));
+    b = batch_new(n, cfg[0]->phys.K, opts);
     if (!b) {
-        fail(-2, "out of host memory");
         return NULL;
-    }
-    b->n = n;
-    b->K = cfg[0]->phys.K;
-    b->batch = (opts && opts->graph_batch > 0) ? opts->graph_batch : 30;
-    b->sims = (FcaHipSim **)calloc((size_t)n, sizeof(FcaHipSim *));
-    b->active = (char *)calloc((size_t)n, 1);
-    b->last_step = (int *)calloc((size_t)n, sizeof(int));
-    b->k_done = (long long *)calloc((size_t)n, sizeof(long long));
-    devs = (FcaDev **)calloc((size_t)n, sizeof(FcaDev *));
-    if (!b->sims || !b->active || !b->last_step || !b->k_done || !devs) {
-        fail(-2, "out of host memory");
-        goto fail_out;
-    }
-    if (hipStreamCreate(&b->st) != hipSuccess) {
-        fail(-2, "hipStreamCreate failed");
-        goto fail_out;
     }
     for (i = 0; i < n; i++) {
         b->sims[i] = sim_create(g[i], cfg[i], &acc[i], opts, b->st);
         if (!b->sims[i]) {
-            char keep[512];
-            snprintf(keep, sizeof(keep), "scenario %d: %s", i, g_err);
-            memcpy(g_err, keep, sizeof(keep));
-            goto fail_out;
+            scenario_error(i);
+            fca_hip_batch_close(b);
+            return NULL;
         }
-        b->cap_wg += b->sims[i]->d.nblocks;
-        b->active[i] = 1;
-        b->last_step[i] = acc[i].step;
+        batch_add(b, i, acc[i].step);
     }
-    /* one pool for every controller record, so a single copy reads them all */
-    if (hipMalloc((void **)&b->ctrl_pool, (size_t)n * 2 * sizeof(FcaDevCtrl)) != hipSuccess ||
-        hipHostMalloc((void **)&b->h_ctrl, (size_t)n * 2 * sizeof(FcaDevCtrl),
-                      hipHostMallocDefault) != hipSuccess) {
-        fail(-2, "controller pool allocation failed");
-        goto fail_out;
+    return batch_finish(b);
+}
+
+FcaHipBatch *fca_hip_batch_open_crops(const FcaHipBase *base, int n, const FcaHipCrop *crops,
+                                      const FcaRunCfg *const *cfg, const FcaHipOpts *opts)
+{
+    FcaHipBatch *b;
+    int i;
+    g_err[0] = 0;
+    if (!base || n <= 0 || !crops || !cfg) {
+        fail(-1, "bad arguments");
+        return NULL;
+    }
+    if (opts && opts->kernel != FCA_HIP_FUSED) {
+        fail(-1, "the batched session runs the fused step only");
+        return NULL;
     }
     for (i = 0; i < n; i++) {
-        FcaDev *d = &b->sims[i]->d;
-        if (hipMemcpy(&b->ctrl_pool[2 * i], d->ctrl, 2 * sizeof(FcaDevCtrl),
-                      hipMemcpyDeviceToDevice) != hipSuccess) {
-            fail(-2, "controller pool copy failed");
-            goto fail_out;
+        if (!cfg[i] || cfg[i]->phys.K != cfg[0]->phys.K) {
+            fail(-1, "scenario %d: missing, or K differs from scenario 0", i);
+            return NULL;
         }
-        d->ctrl = &b->ctrl_pool[2 * i]; /* the session's own pair stays unused in its arena */
-        devs[i] = d;
+        if (cfg[i]->n_force_days > 0) {
+            fail(-1, "scenario %d: multi-day forcing is not supported in a batch", i);
+            return NULL;
+        }
     }
-    b->tab = fca_k_batch_table((const FcaDev *const *)devs, n);
-    if (!b->tab) {
-        fail(-2, "scenario table upload failed");
-        goto fail_out;
+    b = batch_new(n, cfg[0]->phys.K, opts);
+    if (!b) {
+        return NULL;
     }
-    if (hipMalloc((void **)&b->wgs_dev, (size_t)b->cap_wg * 2 * sizeof(int)) != hipSuccess ||
-        hipHostMalloc((void **)&b->wgs_host, (size_t)b->cap_wg * 2 * sizeof(int),
-                      hipHostMallocDefault) != hipSuccess) {
-        fail(-2, "workgroup list allocation failed");
-        goto fail_out;
+    for (i = 0; i < n; i++) {
+        b->sims[i] = sim_create_crop(base, &crops[i], cfg[i], opts, b->st);
+        if (!b->sims[i]) {
+            scenario_error(i);
+            fca_hip_batch_close(b);
+            return NULL;
+        }
+        batch_add(b, i, 0);
     }
-    if (batch_list(b) != 0) {
-        goto fail_out;
-    }
-    free(devs);
-    return b;
-fail_out:
-    free(devs);
-    fca_hip_batch_close(b);
-    return NULL;
+    return batch_finish(b);
 }
 
 int fca_hip_batch_nscen(const FcaHipBatch *b)

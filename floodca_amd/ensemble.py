@@ -81,6 +81,87 @@ def build(base: Grid, sc: Scenario, prec: str = "f32"):
     return g, cfg
 
 
+def classes(base: Grid) -> np.ndarray:
+    """Land-cover class of every cell of base (its roughness's index in
+    MANNING_CLASSES), as build() recovers it."""
+    cls = np.searchsorted(MANNING_CLASSES.astype(base.n.dtype), base.n)
+    return cls.clip(0, MANNING_CLASSES.size - 1).astype(np.uint8)
+
+
+def crop_of(sc: Scenario, prec: str = "f32") -> dict:
+    """The scenario as an FcaHipCrop: the same window, per-class roughness
+    and storm depth as build() gives its grid, so a batch of crops built on
+    the device reproduces a batch of built grids bit for bit."""
+    dt = A.abi(prec).dtype
+    return dict(r0=sc.r0, c0=sc.c0, nrows=sc.side, ncols=sc.side,
+                rday=float(dt.type(0.05 * sc.factor * sc.dur_h)),
+                n_class=[float(v) for v in (MANNING_CLASSES * sc.nscale).astype(dt)])
+
+
+def config_of(sc: Scenario, dx: float, prec: str = "f32"):
+    """The scenario's run configuration (as build())."""
+    a = A.abi(prec)
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, dx)
+    cfg.hyeto = A.FCA_HYETO_TRIANGLE
+    cfg.D_h, cfg.phi = sc.dur_h, 0.5
+    cfg.T_sim = sc.dur_h * 3600.0
+    cfg.steps = 0
+    return cfg
+
+
+def device_base(base: Grid, prec: str = "f32"):
+    """base's bed, mask and classes resident on the device."""
+    from .hip import DeviceBase
+    return DeviceBase(base, classes(base), prec=prec)
+
+
+def run_batched_crops(dbase, scens, dx: float, prec="f32", strip_rows=0, phases=None,
+                      run_wall=None, t_sim_cap=None):
+    """One batch of crop scenarios built on the device from dbase (a
+    DeviceBase): returns (seconds, per-scenario (flooded, max depth, steps)),
+    the time covering the scenario table, the run, the reductions and the
+    release, as run_batched."""
+    t0 = time.perf_counter()
+    res = []
+    cfgs = [config_of(s, dx, prec) for s in scens]
+    if t_sim_cap is not None:
+        cfgs = [_with_tsim(c, min(float(c.T_sim), t_sim_cap)) for c in cfgs]
+    b = Batch.from_crops(dbase, [crop_of(s, prec) for s in scens], cfgs, prec=prec,
+                         strip_rows=strip_rows)
+    try:
+        t1 = time.perf_counter()
+        b.run()
+        t2 = time.perf_counter()
+        for i in range(len(scens)):
+            fl, md = b.flooded(i, THRESHOLDS)
+            res.append((fl, md, b.steps(i)))
+        t3 = time.perf_counter()
+        if run_wall is not None:
+            run_wall.append(b.wall)
+    finally:
+        b.close()
+    t4 = time.perf_counter()
+    if phases is not None:
+        phases.update(open=t1 - t0, run=t2 - t1, results=t3 - t2, close=t4 - t3)
+    return t4 - t0, res
+
+
+def run_alone_crops(dbase, scens, dx: float, prec="f32"):
+    """Each crop scenario in a session of its own, one after another, built
+    on the device as run_batched_crops builds them: returns (seconds,
+    per-scenario results)."""
+    t0 = time.perf_counter()
+    res = []
+    for s in scens:
+        with Batch.from_crops(dbase, [crop_of(s, prec)], [config_of(s, dx, prec)], prec=prec,
+                              strip_rows=strip_alone(s.side, s.side)) as b:
+            b.run()
+            fl, md = b.flooded(0, THRESHOLDS)
+            res.append((fl, md, b.steps(0)))
+    return time.perf_counter() - t0, res
+
+
 def acc_start(prec: str, g: Grid, cfg):
     """Accumulators of a run from g's state (fca_acc_init, model.c:102-109):
     V0 = V = sum of LAND h*A in double, force_day = -1, the rest zero."""
@@ -165,6 +246,21 @@ def autotune_strip(cases, prec="f32", candidates=(32, 48, 64, 96, 128), sample=2
     for s in candidates:
         wall = []
         run_batched(sub, prec, strip_rows=s, run_wall=wall)
+        timings[s] = wall[0]
+    best = min(timings, key=timings.get)
+    return best, timings
+
+
+def autotune_strip_crops(dbase, scens, dx: float, prec="f32", candidates=(32, 48, 64, 96, 128),
+                         sample=24, seed=0):
+    """autotune_strip for crop scenarios built on the device."""
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(len(scens), size=min(sample, len(scens)), replace=False)
+    sub = [scens[i] for i in idx]
+    timings = {}
+    for s in candidates:
+        wall = []
+        run_batched_crops(dbase, sub, dx, prec, strip_rows=s, run_wall=wall, t_sim_cap=1200.0)
         timings[s] = wall[0]
     best = min(timings, key=timings.get)
     return best, timings

@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from .abi import FCA_HIP_FUSED, abi
-from .build import lib_path
+from .build import lib_path, load_hip_library
 
 _LIBS: dict[str, "Backend"] = {}
 
@@ -30,7 +30,7 @@ class Backend:
             raise HipError(f"{path} is not built: run python -m floodca_amd.build")
         self.prec = prec
         self.a = a = abi(prec)
-        self.lib = L = C.CDLL(path)
+        self.lib = L = load_hip_library(path)
         G, Cf, Ac, Tm = (C.POINTER(a.FcaGrid), C.POINTER(a.FcaRunCfg), C.POINTER(a.FcaAcc),
                          C.POINTER(a.FcaTimers))
         P = C.POINTER(a.real)
@@ -72,6 +72,11 @@ class Backend:
             "fca_hip_batch_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                                C.POINTER(C.c_double)]),
             "fca_hip_batch_close": (None, [C.c_void_p]),
+            "fca_hip_base_open": (C.c_void_p, [G, C.c_void_p]),
+            "fca_hip_base_close": (None, [C.c_void_p]),
+            "fca_hip_batch_open_crops": (C.c_void_p, [C.c_void_p, C.c_int,
+                                                      C.POINTER(a.FcaHipCrop), C.POINTER(Cf),
+                                                      C.POINTER(a.FcaHipOpts)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -227,6 +232,41 @@ class Ensemble:
             pass
 
 
+class DeviceBase:
+    """fca_hip_base_*: a grid's bed, mask and land-cover classes resident on
+    the device, the source of crop scenarios (Batch.from_crops)."""
+
+    def __init__(self, grid, cls, prec="f32"):
+        self.b = backend(prec)
+        a = self.b.a
+        self.cls = np.ascontiguousarray(cls, dtype=np.uint8)
+        if self.cls.shape != grid.shape:
+            raise ValueError("one class per cell")
+        self._keep = grid if grid.z.dtype == a.dtype else grid.astype(a.dtype)
+        self._g = self._keep.fca_grid(a)
+        self.h = self.b.lib.fca_hip_base_open(C.byref(self._g), self.cls.ctypes.data)
+        if not self.h:
+            raise HipError(f"fca_hip_base_open: {self.b.error()}")
+        self.shape = grid.shape
+
+    def close(self):
+        if self.h:
+            self.b.lib.fca_hip_base_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """fca_hip_batch_*: scenarios resident at once, advanced together."""
 
@@ -249,6 +289,41 @@ class Batch:
             raise HipError(f"fca_hip_batch_open: {self.b.error()}")
         self.n = n
 
+    @classmethod
+    def from_crops(cls, base: "DeviceBase", crops, cfgs, prec="f32", **kw):
+        """A batch of crop scenarios built on the device from base: crops is
+        a list of FcaHipCrop (or dicts with r0, c0, nrows, ncols, rday,
+        n_class), cfgs one FcaRunCfg each. The scenarios hold no host grids,
+        so read() needs a grid to copy into."""
+        self = cls.__new__(cls)
+        self.b = backend(prec)
+        self.a = a = self.b.a
+        n = len(crops)
+        if n == 0 or n != len(cfgs):
+            raise ValueError("need one cfg per crop")
+        self.grids = [None] * n
+        self.opts = opts(prec, **kw)
+        cr = (a.FcaHipCrop * n)()
+        for i, c in enumerate(crops):
+            if isinstance(c, dict):
+                cr[i].r0, cr[i].c0 = c["r0"], c["c0"]
+                cr[i].nrows, cr[i].ncols = c["nrows"], c["ncols"]
+                cr[i].rday = c["rday"]
+                for k, v in enumerate(c["n_class"]):
+                    cr[i].n_class[k] = v
+            else:
+                cr[i] = c
+        self._crops = cr
+        self._base = base
+        self._c = list(cfgs)
+        self._cp = (C.POINTER(a.FcaRunCfg) * n)(*[C.pointer(x) for x in self._c])
+        self.h = self.b.lib.fca_hip_batch_open_crops(base.h, n, cr, self._cp,
+                                                     C.byref(self.opts))
+        if not self.h:
+            raise HipError(f"fca_hip_batch_open_crops: {self.b.error()}")
+        self.n = n
+        return self
+
     def run(self):
         self.b.check(self.b.lib.fca_hip_batch_run(self.h), "batch run")
 
@@ -260,6 +335,8 @@ class Batch:
         """Copies scenario i's h, hmax, tmax into grid (default: its own) and
         returns its FcaAcc."""
         grid = grid or self.grids[i]
+        if grid is None:
+            raise ValueError("a crop scenario has no host grid: pass one to read into")
         acc = self.a.FcaAcc()
         P = C.POINTER(self.a.real)
         self.b.check(self.b.lib.fca_hip_batch_read(self.h, i, grid.h.ctypes.data_as(P),

@@ -31,7 +31,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .build import nn_lib_path
+from .build import load_hip_library, nn_lib_path
 
 
 class NnError(RuntimeError):
@@ -107,7 +107,7 @@ class Lib:
         path = path or nn_lib_path()
         if not os.path.exists(path):
             raise NnError(f"{path} is not built: run python -m floodca_amd.build")
-        self.lib = ct.CDLL(path)
+        self.lib = load_hip_library(path)
         for name, args in SIGS.items():
             f = getattr(self.lib, name)
             f.restype, f.argtypes = ct.c_int, args

@@ -156,12 +156,13 @@ class Dataset:
 
 
 def ca_targets(base: Grid, scens, R, T, W, row0=0, batch=512, prec="f32", strip_rows=0,
-               timing=None, log=None):
+               timing=None, log=None, dbase=None):
     """Runs scens through the batched CA (chunks of `batch`) and writes their
     targets into rows row0.. of T and W (device): hmax / storm depth, LAND
     fraction x (depth / D0)^2. Returns the per-scenario CA results
     [(flooded, max depth, steps)]. log, if given, receives one line per
-    chunk."""
+    chunk. With dbase (ensemble.device_base of base) the scenarios are built
+    on the device; otherwise on the host and uploaded."""
     torch = _torch()
     L = nn.lib()
     out = []
@@ -169,10 +170,16 @@ def ca_targets(base: Grid, scens, R, T, W, row0=0, batch=512, prec="f32", strip_
     for c0 in range(0, len(scens), batch):
         chunk = scens[c0:c0 + batch]
         t0 = time.perf_counter()
-        cases = [E.build(base, sc, prec) for sc in chunk]
-        accs = [E.acc_start(prec, g, c) for g, c in cases]
-        b = Batch([g for g, _ in cases], [c for _, c in cases], accs, prec=prec,
-                  strip_rows=strip_rows)
+        if dbase is not None:
+            cases = None
+            b = Batch.from_crops(dbase, [E.crop_of(sc, prec) for sc in chunk],
+                                 [E.config_of(sc, base.dx, prec) for sc in chunk], prec=prec,
+                                 strip_rows=strip_rows)
+        else:
+            cases = [E.build(base, sc, prec) for sc in chunk]
+            accs = [E.acc_start(prec, g, c) for g, c in cases]
+            b = Batch([g for g, _ in cases], [c for _, c in cases], accs, prec=prec,
+                      strip_rows=strip_rows)
         try:
             b.run()
             t1 = time.perf_counter()
@@ -206,7 +213,7 @@ def ca_targets(base: Grid, scens, R, T, W, row0=0, batch=512, prec="f32", strip_
 
 
 def make_dataset(base: Grid, terrain: Terrain, scens, R=256, batch=512, prec="f32",
-                 strip_rows=0, log=None) -> Dataset:
+                 strip_rows=0, log=None, dbase=None) -> Dataset:
     torch = _torch()
     N = len(scens)
     q = R // 4
@@ -215,7 +222,7 @@ def make_dataset(base: Grid, terrain: Terrain, scens, R=256, batch=512, prec="f3
     T = torch.empty((N, R, R), dtype=torch.float32, device="cuda")
     W = torch.empty((N, R, R), dtype=torch.float32, device="cuda")
     timing = {}
-    ca = ca_targets(base, scens, R, T, W, 0, batch, prec, strip_rows, timing, log)
+    ca = ca_targets(base, scens, R, T, W, 0, batch, prec, strip_rows, timing, log, dbase)
     t0 = time.perf_counter()
     smp, S, _ = terrain.upload(scens, R)
     Lf = torch.empty((min(N, 512), R, R), dtype=torch.float32, device="cuda")
@@ -378,7 +385,7 @@ def select(sc: Screen, frac_area=0.02, frac_unc=0.02):
 
 
 def judged_error(net: nn.Net, terrain: Terrain, base: Grid, scens, R=256, batch=512,
-                 strip_rows=0, timing=None, log=None):
+                 strip_rows=0, timing=None, log=None, dbase=None):
     """CA reruns of scens: returns (per-scenario CA results, per-scenario
     (sum Lf*(pred - hmax)^2 in m^2, sum Lf) of the surrogate's prediction
     against the CA's target, per-scenario target moments as row_moments in
@@ -388,7 +395,7 @@ def judged_error(net: nn.Net, terrain: Terrain, base: Grid, scens, R=256, batch=
     q = R // 4
     T = torch.empty((N, R, R), dtype=torch.float32, device="cuda")
     W = torch.empty((N, R, R), dtype=torch.float32, device="cuda")
-    ca = ca_targets(base, scens, R, T, W, 0, batch, "f32", strip_rows, timing, log)
+    ca = ca_targets(base, scens, R, T, W, 0, batch, "f32", strip_rows, timing, log, dbase)
     B = 64
     X = torch.empty((B, q, q, 64), dtype=torch.uint16, device="cuda")
     Lf = torch.empty((B, R, R), dtype=torch.float32, device="cuda")
@@ -465,11 +472,12 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     dx = base.dx
     t0 = time.perf_counter()
     terrain = Terrain(base)
+    dbase = E.device_base(base)
     res["t_terrain_s"] = time.perf_counter() - t0
     scens = E.draw(pc.n_train, pc.seed, base.shape, side=pc.side, dur=pc.dur)
     t0 = time.perf_counter()
     data = make_dataset(base, terrain, scens, pc.R, pc.ca_batch,
-                        log=lambda m: log(f"surrogate: dataset {m}"))
+                        log=lambda m: log(f"surrogate: dataset {m}"), dbase=dbase)
     res["t_dataset_s"] = time.perf_counter() - t0
     res["dataset_ca_s"] = data.t_ca
     log(f"surrogate: dataset of {pc.n_train} CA runs in {res['t_dataset_s']:.1f}s "
@@ -523,7 +531,7 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     t0 = time.perf_counter()
     ca, se, mo = judged_error(net, terrain, base, [cands[i] for i in judged], pc.R,
                               pc.ca_batch, timing=timing,
-                              log=lambda m: log(f"surrogate: rerun {m}"))
+                              log=lambda m: log(f"surrogate: rerun {m}"), dbase=dbase)
     res["t_rerun_s"] = time.perf_counter() - t0
     na = len(top_a)
     res["judged"] = int(len(judged))
@@ -536,7 +544,8 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     # the per-scenario all-CA baseline on a seeded sample of candidates
     rng = np.random.default_rng(pc.seed + 2)
     samp = sorted(rng.choice(pc.n_cand, size=min(pc.n_alone, pc.n_cand), replace=False).tolist())
-    t_alone, _ = E.run_alone([E.build(base, cands[i]) for i in samp])
+    t_alone, _ = E.run_alone_crops(dbase, [cands[i] for i in samp], base.dx)
+    dbase.close()
     res["alone_s_per_scenario"] = t_alone / len(samp)
     res["all_ca_alone_scenarios_per_h"] = 3600.0 / res["alone_s_per_scenario"]
     res["all_ca_batched_scenarios_per_h"] = pc.n_train / data.t_ca * 3600.0

@@ -328,6 +328,31 @@ int fca_hip_batch_device_fields(FcaHipBatch *b, int i, void **hmax, void **mask,
                                 int *nc);
 void fca_hip_batch_close(FcaHipBatch *b);
 
+/* Crop scenarios built on the device. A base grid (bed, mask and a land-cover
+ * class per cell) is uploaded once and stays resident; each scenario is a
+ * window of it with its own roughness per class and a uniform Rday (C = 1),
+ * starting dry (h = hmax = tmax = 0; accumulators as fca_acc_init gives them
+ * for a dry grid, force_day = -1). Opening a batch of such scenarios copies
+ * nothing from the host but the scenario table, so the cost of assembling
+ * and uploading every crop is gone. The scenarios are those an FcaGrid
+ * holding the same values would give to fca_hip_batch_open, bit for bit. No
+ * counterpart in the reference, whose ensemble keeps one grid and varies the
+ * forcing (src/hybrid/main_ens_mpi_cuda.c:83-96). */
+#define FCA_HIP_MAX_CLASSES 16
+typedef struct FcaHipBase FcaHipBase;
+/* g supplies z and mask (its other fields are not read); cls holds a class
+ * in [0, FCA_HIP_MAX_CLASSES) per cell. NULL on error (fca_hip_last_error). */
+FcaHipBase *fca_hip_base_open(const FcaGrid *g, const uint8_t *cls);
+void fca_hip_base_close(FcaHipBase *b);
+typedef struct FcaHipCrop {
+    int r0, c0, nrows, ncols;              /* window of the base grid */
+    real rday;                             /* Rday of every cell (C = 1) */
+    real n_class[FCA_HIP_MAX_CLASSES];     /* Manning n of each class */
+} FcaHipCrop;
+/* n scenarios from crops[i] with configuration cfg[i] (same K for all). */
+FcaHipBatch *fca_hip_batch_open_crops(const FcaHipBase *base, int n, const FcaHipCrop *crops,
+                                      const FcaRunCfg *const *cfg, const FcaHipOpts *opts);
+
 #ifdef __cplusplus
 }
 #endif

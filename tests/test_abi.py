@@ -78,3 +78,30 @@ def test_library_loads_and_exports_every_declared_symbol(prec):
     lib = C.CDLL(path)  # loads without a GPU
     for n in names:
         getattr(lib, n)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_crop_table_mirror_matches_the_header(prec, tmp_path):
+    """FcaHipCrop (no counterpart in the reference): the ctypes mirror against
+    a probe compiled from include/floodca_hip.h."""
+    src = tmp_path / "crop.c"
+    exe = tmp_path / "crop"
+    src.write_text("""#include <stddef.h>
+#include <stdio.h>
+#include "floodca_hip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %d\\n", sizeof(FcaHipCrop), offsetof(FcaHipCrop, r0),
+         offsetof(FcaHipCrop, c0), offsetof(FcaHipCrop, nrows), offsetof(FcaHipCrop, ncols),
+         offsetof(FcaHipCrop, rday), offsetof(FcaHipCrop, n_class), FCA_HIP_MAX_CLASSES);
+  return 0;
+}
+""")
+    defs = ["-DFCA_DOUBLE"] if prec == "f64" else []
+    subprocess.run(["gcc", *defs, f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    a = A.abi(prec)
+    T = a.FcaHipCrop
+    assert got == [C.sizeof(T), T.r0.offset, T.c0.offset, T.nrows.offset, T.ncols.offset,
+                   T.rday.offset, T.n_class.offset, A.FCA_HIP_MAX_CLASSES]

@@ -25,7 +25,8 @@ class McArgs(C.Structure):
 
 
 def _lib():
-    lib = C.CDLL(os.path.join(ROOT, "tests", "gpu", "libmathcheck.so"))
+    from floodca_amd.build import load_hip_library
+    lib = load_hip_library(os.path.join(ROOT, "tests", "gpu", "libmathcheck.so"))
     lib.mc_eval.restype = C.c_int
     lib.mc_eval.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(McArgs), C.c_void_p]
     return lib
