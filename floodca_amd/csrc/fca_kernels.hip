@@ -24,6 +24,8 @@
 #include "fca_launch.h"
 #include "fca_walk.h"
 
+static_assert(FCA_TILE_COLS == FCA_OUTC, "host tile width differs from the walk's");
+
 typedef real R;
 typedef CandKey<R>::type Key;
 

@@ -14,6 +14,10 @@ extern "C" {
 #endif
 
 /* Device-resident state of one grid. */
+/* Output columns of one wavefront tile: 64 lanes less 3 halo lanes a side
+ * (FCA_OUTC of the walk). */
+#define FCA_TILE_COLS 58
+
 typedef struct FcaDev {
     int nr, nc, K, kernel;
     real *z, *n, *C, *Rday, *cr, *h[2], *hmax, *tmax;

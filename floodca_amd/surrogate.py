@@ -541,6 +541,11 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     res["judged_rmse_depth_baseline_m"] = rmse(depth_error(c_dep, [cands[i] for i in judged],
                                                            mo))
     res["judged_mean_ca_max_depth_m"] = float(np.mean([r[1] for r in ca]))
+    # what the two flagged sets look like against all candidates
+    for key, rows in (("area", top_a), ("uncertainty", top_u), ("candidates", None)):
+        sel = cands if rows is None else [cands[i] for i in rows]
+        res[f"{key}_mean_side"] = float(np.mean([c.side for c in sel]))
+        res[f"{key}_mean_storm_depth_m"] = float(np.mean([depth(c) for c in sel]))
     # the per-scenario all-CA baseline on a seeded sample of candidates
     rng = np.random.default_rng(pc.seed + 2)
     samp = sorted(rng.choice(pc.n_cand, size=min(pc.n_alone, pc.n_cand), replace=False).tolist())
