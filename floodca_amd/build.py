@@ -5,7 +5,9 @@
 * floodca_amd/libfca_nn.so -- the surrogate network's kernels (hipcc, gfx950).
 * tests/emu/libwalk_emu_{f32,f64}.so -- host emulation of the fused walk for the
   CPU test suite (g++), never loaded by the product.
-* oracle/ -- the checker (oracle/build_oracle.sh).
+* oracle/ -- the checker (oracle/build_oracle.sh), and, where the reference
+  tree is present, the reference's driver around this library
+  (oracle/build_hip_main.sh -> oracle/_ref/floodca_hip_{f32,f64}).
 
 Run ``python -m floodca_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -124,6 +126,13 @@ def build_oracle() -> None:
     _run(["bash", os.path.join(ROOT, "oracle", "build_oracle.sh")])
 
 
+def build_hip_main() -> None:
+    """oracle/_ref/floodca_hip_{f32,f64}: the reference's driver around this
+    library (oracle/build_hip_main.sh; needs the reference tree and the built
+    libraries, else nothing is built)."""
+    _run(["bash", os.path.join(ROOT, "oracle", "build_hip_main.sh")])
+
+
 def build_all(verbose: bool = True) -> list[str]:
     if shutil.which(os.path.join(ROCM, "bin", "hipcc")) is None:
         raise RuntimeError(f"hipcc not found under {ROCM}")
@@ -133,6 +142,7 @@ def build_all(verbose: bool = True) -> list[str]:
         futs = [ex.submit(f, p) for f, p in jobs] + [ex.submit(build_oracle),
                                                      ex.submit(build_mathcheck)]
         outs = [f.result() for f in futs]
+    build_hip_main()  # links against the libraries just built
     if verbose:
         for o in outs:
             if o:

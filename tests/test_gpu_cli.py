@@ -1,0 +1,72 @@
+"""The library behind the reference's own command-line driver (GPU).
+
+oracle/_ref/floodca_hip_{f32,f64} is floodca_amd/csrc/main_hip.c compiled
+with the reference's src/common (CLI, grid-dir loading, warm start,
+accumulators, CSV report) and linked against libfloodca_hip*.so -- the
+executable a maintainer adds beside floodca_cuda. Run on a grid dir written
+for the reference, it must leave the hmax and h_final rasters the oracle
+computes from the same dir (FP32 bit for bit, FP64 within 1e-12 m), and
+append a CSV row with the reference serial driver's columns. Built by
+__graft_entry__.build() where the reference tree exists; skipped otherwise.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from floodca_amd import abi as A
+from floodca_amd import synth
+from floodca_amd.grid import load_dir, read_bin, write_dir
+from oracle import Oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+pytestmark = pytest.mark.gpu
+
+
+def _exe(name):
+    p = os.path.join(REF, name)
+    if not os.path.exists(p):
+        pytest.skip(f"{p} not built (needs the reference tree at build time)")
+    return p
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_reference_driver_runs_the_hip_backend(prec, tmp_path):
+    exe = _exe(f"floodca_hip_{prec}")
+    grid_dir, out_dir = tmp_path / "grid", tmp_path / "out"
+    csv = tmp_path / "runs.csv"
+    g = synth.make_grid(193, 257, seed=21, h0=0.05)
+    write_dir(str(grid_dir), g)
+    print(f"case floodca_hip {prec}", flush=True)
+    r = subprocess.run([exe, "--grid", str(grid_dir), "--tsim", "3600", "--hyeto", "uniform",
+                        "--out", str(out_dir), "--csv", str(csv)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    done = [ln for ln in r.stdout.splitlines() if ln.startswith("done backend=hip")]
+    assert done and "hbm_alg_gbps=" in done[0], r.stdout[-2000:]
+    a = A.abi(prec)
+    go = load_dir(str(grid_dir), dtype=a.dtype)
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, go.dx)
+    cfg.hyeto = A.FCA_HYETO_UNIFORM
+    cfg.T_sim = 3600.0
+    acc, _ = Oracle(prec).run(go, cfg)
+    assert f"steps={acc.step} " in done[0]
+    hmax = read_bin(str(out_dir / "hmax.bin"))
+    h = read_bin(str(out_dir / "h_final.bin"))
+    if prec == "f32":
+        assert np.array_equal(hmax, go.hmax) and np.array_equal(h, go.h)
+    else:
+        assert np.abs(hmax - go.hmax).max() <= 1e-12 and np.abs(h - go.h).max() <= 1e-12
+    # the CSV row has the reference serial driver's columns
+    csv_ref = tmp_path / "serial.csv"
+    subprocess.run([_exe(f"floodca_serial_{prec}"), "--grid", str(grid_dir), "--steps", "2",
+                    "--tsim", "1e9", "--out", str(tmp_path / "serial_out"), "--csv", str(csv_ref)],
+                   capture_output=True, text=True, timeout=300, check=True)
+    ours, theirs = csv.read_text().splitlines(), csv_ref.read_text().splitlines()
+    assert ours[0] == theirs[0]
+    assert len(ours[1].split(",")) == len(theirs[1].split(","))
+    assert ",hip," in ours[1]
