@@ -458,3 +458,43 @@ class _Addr:
 
     def data_ptr(self) -> int:
         return self.a
+
+
+def forward_flops(cfg: NetCfg) -> int:
+    """Multiply-adds x 2 of one sample's forward pass (every GEMM's M*N*K)."""
+    return 2 * sum(ly.hout * ly.hout * ly.cout * ly.k for ly in layers(cfg))
+
+
+def main(argv=None) -> int:
+    """Times the forward pass alone: ``python -m floodca_amd.nn --batch 256
+    --reps 20`` (random weights, zero inputs; the kernels do the same work
+    whatever the values), for profiling the matrix-core GEMMs."""
+    import argparse
+
+    ap = argparse.ArgumentParser(description=main.__doc__.split("\n")[0])
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--reps", type=int, default=20)
+    args = ap.parse_args(argv)
+    torch = _torch()
+    net = Net(NetCfg(), seed=0)
+    q = net.cfg.R // 4
+    X = torch.zeros((args.batch, q, q, 64), dtype=torch.uint16, device="cuda")
+    S = torch.zeros((args.batch, 4), dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        net.forward(X, S, args.batch)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(args.reps):
+        net.forward(X, S, args.batch)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.reps
+    fl = forward_flops(net.cfg) * args.batch
+    print(f"forward: {ms:.3f} ms per batch of {args.batch}, {fl / (ms * 1e-3) / 1e12:.1f} TFLOP/s "
+          f"({net.n} parameters, {forward_flops(net.cfg) / 1e9:.3f} GFLOP per sample)", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
