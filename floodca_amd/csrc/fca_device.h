@@ -412,12 +412,47 @@ FCA_HDI double div_step_cr(double num, double den, double rden)
     return num / den;
 }
 
+/* Whether num/den for this step's den may be taken as the two-fma refinement
+ * of num*RN(1/den) without a certificate: den in [2^-40, 2^40]. With
+ * rden = RN(1/den) and q0 = RN(num*rden), q = fma(fma(-q0, den, num), rden,
+ * q0) is the correctly rounded quotient for every divisor and dividend
+ * significand (Markstein's refinement; checked on the GPU over all
+ * 2^23 x 2^23 significand pairs, tests/test_gpu_math.py), and dividends in
+ * [2^-80, 2^80] keep every intermediate a normal number. */
+FCA_HD int step_den_fast(float den)
+{
+    return den >= 0x1p-40f && den <= 0x1p40f;
+}
+FCA_HD int step_den_fast(double) { return 0; }
+
+/* div_step_cr with the step's den_fast flag: the certificate only where the
+ * flag is off or num is out of range. */
+FCA_HDI float div_step_fast(float num, float den, float rden, int den_fast)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float an = __builtin_fabsf(num);
+    if (den_fast && an >= 0x1p-80f && an <= 0x1p80f) {
+        const float q0 = num * rden;
+        return __builtin_fmaf(__builtin_fmaf(-q0, den, num), rden, q0);
+    }
+    return div_step_cr(num, den, rden);
+#else
+    (void)rden, (void)den_fast;
+    return num / den;
+#endif
+}
+FCA_HDI double div_step_fast(double num, double den, double rden, int)
+{
+    return div_step_cr(num, den, rden);
+}
+
 /* The level-equalisation cap Qcap = |dL|*A / ((2K)*dt) (physics.h:113-121). */
 template <class R>
-FCA_HDI R level_cap(R qmag, R adL, R dt, R den, R rden, const PhysK<R> &p, int &cap)
+FCA_HDI R level_cap(R qmag, R adL, R dt, R den, R rden, const PhysK<R> &p, int &cap,
+                    int den_fast = 0)
 {
     if (dt > (R)0 && p.K > 0) {
-        const R qcap = div_step_cr(adL * p.A, den, rden);
+        const R qcap = div_step_fast(adL * p.A, den, rden, den_fast);
         if (qcap < qmag) {
             cap = 1;
             qmag = qcap;
@@ -495,7 +530,7 @@ FCA_HD R face_flux(R za, R ha, R na, int ma, R zb, R hb, R nb, int mb, R d, R rd
  * the wall, sea and clamp terms (all inactive) left out. */
 template <class R>
 FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, R rden,
-                    const PhysK<R> &p, int &cap)
+                    int den_fast, const PhysK<R> &p, int &cap)
 {
     const R dL = la - lb;
     /* |dL| as a magnitude: where face_flux keeps a -0 this gives +0, and
@@ -509,7 +544,7 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
         R v = rcp_in_range(nf) * cbrt_sq_big(hf) * sqrt_in_range(div_c_big(adL, d, rd));
         v = crit_cap<R, 1>(v, hf, p);
         qmag = v * hf * p.w;
-        qmag = level_cap<R>(qmag, adL, dt, den, rden, p, cap);
+        qmag = level_cap<R>(qmag, adL, dt, den, rden, p, cap, den_fast);
         qmag = (dL > (R)0) ? qmag : -qmag;
     }
     return qmag;
@@ -595,6 +630,7 @@ struct StepVals {
     double t, F, t_new;
     R dt, dt_base, tmax_val, rain_f, den, rden;
     int active, hit, need_cand, step;
+    int den_fast; /* den admits div_step_fast (step_den_fast) */
 };
 
 /* rain.c:13-28 */
@@ -760,6 +796,7 @@ FCA_HD void ctrl_eval(const FcaDevCtrl &in, R cand, const FcaCtlParams &c, const
     sv.tmax_val = (R)(in.t + (double)dt);
     sv.den = (R)(2 * p.K) * dt;
     sv.rden = (R)1 / sv.den;
+    sv.den_fast = step_den_fast(sv.den);
     sv.t_new = hit ? t_next : in.t + (double)dt;
     sv.need_cand = ((in.step + 1) % (c.U > 0 ? c.U : 1)) == 0;
     nx.t = sv.t_new;

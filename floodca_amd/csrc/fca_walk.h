@@ -291,6 +291,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
     const PhysK<R> &p = a.p;
     const StepVals<R> &sv = a.sv;
     const R dt = sv.dt, den = sv.den, rden = sv.rden, A = p.A;
+    const int dfast = sv.den_fast;
     const int rowA = r < nr;                            /* row r exists */
     const int outA = r >= w.r_out0 && r < w.r_out1;    /* row r is written here */
     const int qB = r - 1, qC = r - 2, qD = r - 3;
@@ -309,17 +310,17 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
         lp.each([&](L &s, int) {
             int cap, ct = 0;
             s.qx0 = face_land<R>(s.z0, s.lv0, s.n0, s.xz, s.xl, s.xn, p.dx, p.rdx, dt, den, rden,
-                                 p, cap);
+                                 dfast, p, cap);
             ct += outA ? cap : 0;
             s.qy1 = face_land<R>(s.z1, s.lv1, s.n1, s.z0, s.lv0, s.n0, p.dx, p.rdx, dt, den,
-                                 rden, p, cap);
+                                 rden, dfast, p, cap);
             ct += outB ? cap : 0;
             if (K8) {
                 s.qd1_1 = face_land<R>(s.z1, s.lv1, s.n1, s.xz, s.xl, s.xn, p.ddiag, p.rddiag,
-                                       dt, den, rden, p, cap);
+                                       dt, den, rden, dfast, p, cap);
                 ct += outB ? cap : 0;
                 s.qd2_1 = face_land<R>(s.xq, s.xl1, s.xs, s.z0, s.lv0, s.n0, p.ddiag, p.rddiag,
-                                       dt, den, rden, p, cap);
+                                       dt, den, rden, dfast, p, cap);
                 ct += outB ? cap : 0;
             }
             if (s.outcol) {

@@ -75,9 +75,57 @@ __global__ void k_eval(int kind, uint32_t first, uint32_t count, McArgs a, float
         y = (!need && v > thr) ? 1.0f : (need ? 0.5f : 0.0f);
         break;
     }
+    case 16: y = div_step_fast(x, a.d, a.rd, step_den_fast(a.d)); break;
     default: y = sqrt_rough(x); break;
     }
     out[i] = y;
+}
+
+/* For each divisor significand m = first + i (divisor 1.m in [1, 2)): the
+ * number of dividend significands x in [1, 2) for which the two-fma
+ * refinement of x*RN(1/d) is not the correctly rounded x/d. A refinement q
+ * is correctly rounded iff its exact remainder r = x - q*d (one fma) is
+ * below d times half the gap from q to its neighbour on r's side (no
+ * quotient of two floats lies halfway between two floats, so the strict
+ * test decides it). */
+__global__ void k_div_pairs(uint32_t first, uint32_t count, uint32_t *fails)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) {
+        return;
+    }
+    const float d = __uint_as_float(0x3f800000u | ((first + (uint32_t)i) & 0x7FFFFFu));
+    const float r = 1.0f / d;
+    uint32_t bad = 0;
+    for (uint32_t m = 0; m < (1u << 23); m++) {
+        const float x = __uint_as_float(0x3f800000u | m);
+        const float q0 = x * r;
+        const float q = __builtin_fmaf(__builtin_fmaf(-q0, d, x), r, q0);
+        const float rem = __builtin_fmaf(-q, d, x);
+        /* the rounding boundary on the side of the exact quotient: half the
+         * gap to q's neighbour there (the gap below a power of two is half
+         * the gap above it) */
+        const float ulp = (rem < 0.0f) ? q - __uint_as_float(__float_as_uint(q) - 1u)
+                                       : __uint_as_float(__float_as_uint(q) + 1u) - q;
+        bad += !(__builtin_fabsf(rem) < 0.5f * ulp * d);
+    }
+    fails[i] = bad;
+}
+
+extern "C" int mc_div_pairs(uint32_t first, uint32_t count, uint32_t *out)
+{
+    uint32_t *dv = nullptr;
+    hipError_t e = hipMalloc((void **)&dv, (size_t)count * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        return (int)e;
+    }
+    hipLaunchKernelGGL(k_div_pairs, dim3((count + 255) / 256), dim3(256), 0, 0, first, count, dv);
+    e = hipGetLastError();
+    if (e == hipSuccess) {
+        e = hipMemcpy(out, dv, (size_t)count * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(dv);
+    return (int)e;
 }
 
 /* Evaluates kind over [first, first+count) into host out. Returns 0 or a HIP error. */

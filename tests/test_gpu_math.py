@@ -265,3 +265,49 @@ def test_stage_d_log_bound_never_skips_a_faster_cell(n, ax, k):
         assert not np.any(got == 1.0), f"skipped a faster cell (n={n}, ax={ax}, k={k})"
         if k < 0 and ax > 0:
             assert np.all(got == 0.5)
+
+
+def _div_pairs_lib():
+    lib = _lib()
+    lib.mc_div_pairs.restype = C.c_int
+    lib.mc_div_pairs.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p]
+    return lib
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_two_fma_division_is_exact_for_every_significand_pair():
+    """Markstein's refinement without a certificate (div_step_fast): over all
+    2^23 divisor significands x all 2^23 dividend significands, every
+    quotient is correctly rounded (the all-ones divisor significand, the
+    hardest case for a reciprocal, included)."""
+    lib = _div_pairs_lib()
+    print("exhaustive division pairs", flush=True)
+    bad = []
+    step = 1 << 20
+    for first in range(0, 1 << 23, step):
+        out = np.empty(step, dtype=np.uint32)
+        rc = lib.mc_div_pairs(first, step, out.ctypes.data)
+        assert rc == 0, f"mc_div_pairs failed: HIP error {rc}"
+        bad += (np.flatnonzero(out) + first).tolist()
+        print(f"divisors {first:#x}..{first + step - 1:#x}: {np.count_nonzero(out)} failing",
+              flush=True)
+    assert bad == []
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("den", [480.0, 960.0, float(np.float32(8.0) * np.float32(52.371)),
+                                 float(np.float32(16.0) * np.float32(0.0137)),
+                                 float(np.float32(8.0) * np.float32(1.9999999))])
+def test_step_division_with_the_fast_flag_is_correctly_rounded(den):
+    """div_step_fast as the LAND form calls it, against IEEE division for
+    every dividend in [2^-80, 2^80]."""
+    lib = _lib()
+    d = np.float32(den)
+    rd = np.float32(1) / d
+    for first, n in _range(2.0 ** -80, 2.0 ** 80):
+        got = dev(lib, 16, first, n, d=float(d), rd=float(rd))
+        x = bits(first, n).view(np.float32)
+        idx = same(got, x / d)
+        assert idx.size == 0, f"x/{d} mismatch at {hex(first + int(idx[0]))}"
