@@ -141,9 +141,44 @@ def cpu_baseline(seconds: float, prec: str):
             o.run(gg, cfg)
             wall = time.time() - t
             kind = "port"
-    return {"value": n_land * steps / wall, "unit": "cell-updates/s", "cores": 1, "kind": kind,
-            "sample": f"512^2 grid, {n_land} LAND cells, {steps} steps from a dry start with "
-                      f"50 mm/h rain, FP32, serial, wall {wall:.1f} s"}
+    out = {"value": n_land * steps / wall, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+           "sample": f"512^2 grid, {n_land} LAND cells, {steps} steps from a dry start with "
+                     f"50 mm/h rain, FP32, serial, wall {wall:.1f} s"}
+    omp = cpu_baseline_omp(seconds, prec)
+    if omp:
+        out["omp"] = omp
+    return out
+
+
+def cpu_baseline_omp(seconds: float, prec: str, size: int = 2048):
+    """The reference OpenMP build (oracle/_ref/floodca_omp_*, stock -O3) on
+    a size^2 grid of the same generator with every host core the process may
+    use (OMP_NUM_THREADS, else the affinity mask), threads bound close to
+    cores as the reference's job scripts set them (slurm/env.sh:35-36), timed
+    for about `seconds`. None when the build is absent."""
+    exe = os.path.join(ROOT, "oracle", "_ref", f"floodca_omp_{prec}")
+    if not os.path.exists(exe):
+        return None
+    cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    g = synth.make_grid(size, size, seed=20261203)
+    with tempfile.TemporaryDirectory() as td:
+        write_dir(td, g)
+
+        def run(steps):
+            out = subprocess.run([exe, "--grid", td, "--steps", str(steps), "--tsim", "1e9",
+                                  "--hyeto", "uniform", "--threads", str(cores), "--no-output",
+                                  "--csv", ""], capture_output=True, text=True, check=True,
+                                 env=env).stdout
+            return float(re.search(r"wall=([0-9.eE+-]+)", out).group(1))
+        w0 = run(10)
+        steps = max(10, int(seconds / max(w0 / 10, 1e-6)))
+        wall = run(steps)
+    return {"value": g.n_land * steps / wall, "unit": "cell-updates/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"{size}^2 grid, {g.n_land} LAND cells, {steps} steps from a dry start "
+                      f"with 50 mm/h rain, FP32, OpenMP on {cores} threads, wall {wall:.1f} s"}
 
 
 def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261204):
@@ -286,7 +321,8 @@ def surrogate_bench(base, args):
     from floodca_amd import nn
     from floodca_amd import surrogate as SG
     pc = SG.PipelineCfg(n_train=args.sur_train, n_val=max(64, args.sur_train // 16),
-                        n_cand=args.sur_cand, epochs=args.sur_epochs, n_alone=args.sur_alone)
+                        n_cand=args.sur_cand, epochs=args.sur_epochs, n_alone=args.sur_alone,
+                        train_sigma=args.sur_train_sigma)
     res = SG.run_pipeline(base, pc, log=lambda m: print(f"[bench] {m}", file=sys.stderr,
                                                        flush=True))
     o = res.pop("_objects")
@@ -349,6 +385,8 @@ def main() -> int:
     ap.add_argument("--sur-cand", type=int, default=20000,
                     help="candidates it screens (the full case: 100000)")
     ap.add_argument("--sur-epochs", type=int, default=30)
+    ap.add_argument("--sur-train-sigma", type=float, default=0.5,
+                    help="spread of the training storms' log intensity")
     ap.add_argument("--sur-alone", type=int, default=48,
                     help="candidates timed one session each (the all-CA baseline)")
     ap.add_argument("--no-4096", dest="grid4096", action="store_false",

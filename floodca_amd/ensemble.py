@@ -44,7 +44,7 @@ class Scenario:
     nscale: np.ndarray
 
 
-def draw(n: int, seed: int, big: tuple, side=(256, 2048), dur=(2.0, 12.0)):
+def draw(n: int, seed: int, big: tuple, side=(256, 2048), dur=(2.0, 12.0), sigma: float = 0.5):
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -52,7 +52,7 @@ def draw(n: int, seed: int, big: tuple, side=(256, 2048), dur=(2.0, 12.0)):
         s = min(s, big[0], big[1])
         out.append(Scenario(idx=i, r0=int(rng.integers(0, big[0] - s + 1)),
                             c0=int(rng.integers(0, big[1] - s + 1)), side=s,
-                            factor=float(rng.lognormal(0.0, 0.5)),
+                            factor=float(rng.lognormal(0.0, sigma)),
                             dur_h=float(rng.uniform(dur[0], dur[1])),
                             nscale=rng.uniform(0.8, 1.2, MANNING_CLASSES.size)))
     return out

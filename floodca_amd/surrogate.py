@@ -445,6 +445,10 @@ class PipelineCfg:
     seed: int = 20261206
     ca_batch: int = 512
     screen_batch: int = 256
+    # spread of the training storms' log intensity (the candidates keep the
+    # generator's 0.5): a wider training draw covers the deep-storm tail the
+    # spread ranking flags
+    train_sigma: float = 0.5
 
 
 def _spearman(a, b) -> float:
@@ -474,7 +478,8 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     terrain = Terrain(base)
     dbase = E.device_base(base)
     res["t_terrain_s"] = time.perf_counter() - t0
-    scens = E.draw(pc.n_train, pc.seed, base.shape, side=pc.side, dur=pc.dur)
+    scens = E.draw(pc.n_train, pc.seed, base.shape, side=pc.side, dur=pc.dur,
+                   sigma=pc.train_sigma)
     t0 = time.perf_counter()
     data = make_dataset(base, terrain, scens, pc.R, pc.ca_batch,
                         log=lambda m: log(f"surrogate: dataset {m}"), dbase=dbase)
