@@ -344,6 +344,49 @@ FCA_HD float cbrt_sq_big(float hf)
 }
 FCA_HD double cbrt_sq_big(double hf) { return cbrt_sq(hf); }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+/* Whether the low 29 bits of a double estimate y (those below float
+ * precision) lie within FCA_CBRT_GUARD double ulps of the float rounding
+ * midpoint, i.e. (float)y might not be the correctly rounded value. */
+__device__ inline int near_float_midpoint(double y)
+{
+    uint64_t bits;
+    __builtin_memcpy(&bits, &y, 8);
+    const int lo = (int)((uint32_t)bits & 0x1FFFFFFFu) - 0x10000000;
+    return !(lo > FCA_CBRT_GUARD || lo < -FCA_CBRT_GUARD);
+}
+
+/* cbrt_sq_big without branches: the correctly rounded hf^(2/3) for
+ * 2^-15 <= hf <= 2^15 unless *fix is set (near a midpoint, or hf out of that
+ * range), where the caller must take cbrt_sq_big. */
+__device__ inline float cbrt_sq_cert(float hf, int &fix)
+{
+    const double y1 = cbrt_sq_newton(hf, (double)hf * (double)hf);
+    fix |= near_float_midpoint(y1) | !(hf <= FCA_CBRT_HI);
+    return (float)y1;
+}
+
+/* (float)sqrt((double)g*(double)h), the critical speed, without branches:
+ * x = g*h is exact in double; s = the hardware float sqrt of (float)x is
+ * within 2^-22 of sqrt(x), one Newton step y = s + (x - s^2)/(2s) (the
+ * residual exact in one fma, the reciprocal the hardware's) brings it within
+ * 2^-44, and y is the correctly rounded float unless it lies within 2^-38 of
+ * a float midpoint, where *fix is set and the caller must take crit_speed.
+ * x within [2^-100, 2^100] keeps every step normal (the LAND form has
+ * h >= 2^-15 and g in [2^-80, 2^80]). Checked against the double expression
+ * for every h in [2^-15, 2^15] (tests/test_gpu_math.py). */
+__device__ inline float crit_speed_cert(float g, float h, int &fix)
+{
+    const double x = (double)g * (double)h;
+    const float s = __builtin_amdgcn_sqrtf((float)x);
+    const double sd = (double)s;
+    const double r = __builtin_fma(-sd, sd, x);
+    const double y = __builtin_fma(r, (double)__builtin_amdgcn_rcpf(2.0f * s), sd);
+    fix |= near_float_midpoint(y);
+    return (float)y;
+}
+#endif
+
 FCA_HD float sqrt_via_double(float x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -529,7 +572,7 @@ FCA_HD R face_flux(R za, R ha, R na, int ma, R zb, R hb, R nb, int mb, R d, R rd
  * for runs where fca_land_form_ok holds: the expressions of face_flux with
  * the wall, sea and clamp terms (all inactive) left out. */
 template <class R>
-FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, R rden,
+FCA_HDI R face_land_exact(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, R rden,
                     int den_fast, const PhysK<R> &p, int &cap)
 {
     const R dL = la - lb;
@@ -548,6 +591,67 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
         qmag = (dL > (R)0) ? qmag : -qmag;
     }
     return qmag;
+}
+
+/* face_land as the LAND form evaluates it. On the device (float) every lane
+ * takes the same straight-line path -- dead faces computed on stand-in
+ * inputs and masked at the end, the cube root and the critical speed in
+ * their certified branch-free forms (cbrt_sq_cert, crit_speed_cert), Qcap
+ * by Markstein's refinement -- and each lane raises a flag where a
+ * certificate fails or an input leaves the ranges those forms assume; one
+ * ballot then sends the flagged lanes, rarely any, through face_land_exact.
+ * A wavefront with no live face skips the arithmetic. The values are
+ * face_land_exact's, bit for bit. */
+template <class R>
+FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, R rden,
+                    int den_fast, const PhysK<R> &p, int &cap)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (sizeof(R) == 4) {
+        const float dL = la - lb;
+        const float adL = __builtin_fabsf(dL);
+        const float hf = ((la > lb) ? la : lb) - ((za > zb) ? za : zb);
+        const int live = !((adL < (float)p.tau_l) | (hf < (float)p.tau_h));
+        cap = 0;
+        if (__builtin_amdgcn_ballot_w64(live) == 0) {
+            return (R)0;
+        }
+        {
+            /* dead lanes compute on 1 m stand-ins, inside every fast range */
+            const float hx = live ? hf : 1.0f, ax = live ? adL : 1.0f;
+            int fix = !(ax <= 0x1p80f) | !den_fast;
+            const float nf = 0.5f * ((float)na + (float)nb);
+            const float q0 = ax * (float)rd;
+            const float sl = __builtin_fmaf(__builtin_fmaf(-q0, (float)d, ax), (float)rd, q0);
+            float v = rcp_in_range(nf) * cbrt_sq_cert(hx, fix) * sqrt_in_range(sl);
+            const float vcr = crit_speed_cert((float)p.g, hx, fix);
+            float qmag, x, c0, qcap;
+            if (v > vcr) {
+                v = vcr;
+            }
+            qmag = v * hx * (float)p.w;
+            x = ax * (float)p.A;
+            fix |= !(x >= 0x1p-80f && x <= 0x1p80f);
+            c0 = x * (float)rden;
+            qcap = __builtin_fmaf(__builtin_fmaf(-c0, (float)den, x), (float)rden, c0);
+            cap = live & (qcap < qmag);
+            qmag = (qcap < qmag) ? qcap : qmag;
+            qmag = (dL > 0.0f) ? qmag : -qmag;
+            if (__builtin_amdgcn_ballot_w64(live & fix) != 0) {
+                FCA_COLD_PATH();
+                if (live & fix) {
+                    int cx;
+                    const R q = face_land_exact<R>(za, la, na, zb, lb, nb, d, rd, dt, den, rden,
+                                                   den_fast, p, cx);
+                    cap = cx;
+                    return q;
+                }
+            }
+            return live ? (R)qmag : (R)0; /* a dead face is +0, as face_land_exact */
+        }
+    }
+#endif
+    return face_land_exact<R>(za, la, na, zb, lb, nb, d, rd, dt, den, rden, den_fast, p, cap);
 }
 
 /* Open-boundary virtual face of a LAND edge cell: z_b = z_a - S_bnd*dx,

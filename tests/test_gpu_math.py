@@ -311,3 +311,44 @@ def test_step_division_with_the_fast_flag_is_correctly_rounded(den):
         x = bits(first, n).view(np.float32)
         idx = same(got, x / d)
         assert idx.size == 0, f"x/{d} mismatch at {hex(first + int(idx[0]))}"
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("g", [9.81, 9.80665, 1.62, 24.79])
+def test_certified_critical_speed_on_every_depth(g):
+    """crit_speed_cert, the LAND form's branch-free critical speed: for every
+    h in [2^-15, 2^15] it either defers (rarely) or equals the reference's
+    (float)sqrt((double)g*(double)h) bit for bit."""
+    lib = _lib()
+    gf = np.float32(g)
+    deferred = total = 0
+    for first, n in _range(2.0 ** -15, 2.0 ** 15):
+        got = dev(lib, 17, first, n, g=float(gf))
+        h = bits(first, n).view(np.float32)
+        ref = np.sqrt(np.float64(gf) * h.astype(np.float64)).astype(np.float32)
+        ok = got >= 0
+        idx = same(got[ok], ref[ok])
+        assert idx.size == 0, f"g={g}: mismatch at h={h[ok][idx[0]]!r}"
+        deferred += int(np.count_nonzero(~ok))
+        total += n
+    assert deferred < total * 2.0 ** -10
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_certified_cube_root_on_every_depth():
+    """cbrt_sq_cert, the LAND form's branch-free hf^(2/3): for every hf in
+    [2^-15, 2^15] it either defers (rarely) or equals glibc's
+    (float)cbrt((double)hf*hf) bit for bit."""
+    lib, o = _lib(), Oracle("f32")
+    deferred = total = 0
+    for first, n in _range(2.0 ** -15, 2.0 ** 15):
+        got = dev(lib, 18, first, n)
+        ok = got >= 0
+        ref = o.libm_range(0, first, n)
+        idx = same(got[ok], ref[ok])
+        assert idx.size == 0, f"mismatch at bits {hex(first + int(np.flatnonzero(ok)[idx[0]]))}"
+        deferred += int(np.count_nonzero(~ok))
+        total += n
+    assert deferred < total * 2.0 ** -10
