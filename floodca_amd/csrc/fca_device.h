@@ -624,10 +624,17 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
             const float q0 = ax * (float)rd;
             const float sl = __builtin_fmaf(__builtin_fmaf(-q0, (float)d, ax), (float)rd, q0);
             float v = rcp_in_range(nf) * cbrt_sq_cert(hx, fix) * sqrt_in_range(sl);
-            const float vcr = crit_speed_cert((float)p.g, hx, fix);
             float qmag, x, c0, qcap;
-            if (v > vcr) {
-                v = vcr;
+            {
+                /* the critical speed only where some lane's v is not clearly
+                 * below it (crit_cap's test): a uniform branch */
+                const float gh = (float)p.g * hx;
+                if (__builtin_amdgcn_ballot_w64(live & !(v * v < gh * (1.0f - 0x1p-16f))) != 0) {
+                    const float vcr = crit_speed_cert((float)p.g, hx, fix);
+                    if (v > vcr) {
+                        v = vcr;
+                    }
+                }
             }
             qmag = v * hx * (float)p.w;
             x = ax * (float)p.A;
