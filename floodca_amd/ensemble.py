@@ -253,14 +253,16 @@ def autotune_strip(cases, prec="f32", candidates=(32, 48, 64, 96, 128), sample=2
 
 def autotune_strip_crops(dbase, scens, dx: float, prec="f32", candidates=(32, 48, 64, 96, 128),
                          sample=24, seed=0):
-    """autotune_strip for crop scenarios built on the device."""
+    """autotune_strip for crop scenarios built on the device, each sample
+    run cut to its first 2 h of simulated time (runs of 20 min took ~10 ms,
+    too short to rank the heights)."""
     rng = np.random.default_rng(seed)
     idx = rng.choice(len(scens), size=min(sample, len(scens)), replace=False)
     sub = [scens[i] for i in idx]
     timings = {}
     for s in candidates:
         wall = []
-        run_batched_crops(dbase, sub, dx, prec, strip_rows=s, run_wall=wall, t_sim_cap=1200.0)
+        run_batched_crops(dbase, sub, dx, prec, strip_rows=s, run_wall=wall, t_sim_cap=7200.0)
         timings[s] = wall[0]
     best = min(timings, key=timings.get)
     return best, timings
