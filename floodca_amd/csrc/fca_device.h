@@ -684,6 +684,49 @@ FCA_HD R limiter_s(R out_vol, R h, R A)
     return (R)1;
 }
 
+/* limiter_s with a uniform branch on the device: the division runs on every
+ * lane when some lane of the wavefront limits, and not at all otherwise, so
+ * no lane mask is saved and restored (the LAND form). Same values. */
+template <class R>
+FCA_HDI R limiter_s_wave(R out_vol, R h, R A)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const R hold = h * A;
+    const int lim = out_vol > hold && out_vol > (R)0;
+    R s = (R)1;
+    if (__builtin_amdgcn_ballot_w64(lim) != 0) {
+        const R q = hold / out_vol;
+        s = lim ? q : s;
+    }
+    return s;
+#else
+    return limiter_s<R>(out_vol, h, A);
+#endif
+}
+
+/* div_c for the LAND form's net/A (the flag of A set, checked by the host):
+ * the two-fma form on every lane, a zero passed through with its sign, and a
+ * correctly rounded division, on the lanes that need it, only when some
+ * dividend lies outside [2^-80, 2^80] (a uniform branch). Same values. */
+FCA_HDI float div_c_wave(float a, float d, float rd)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float aa = __builtin_fabsf(a);
+    const float q0 = a * rd;
+    float q = __builtin_fmaf(__builtin_fmaf(-q0, d, a), rd, q0);
+    const int slow = !((aa >= 0x1p-80f && aa <= 0x1p80f) || aa == 0.0f);
+    q = (aa == 0.0f) ? q0 : q;
+    if (__builtin_amdgcn_ballot_w64(slow) != 0) {
+        const float qd = a / d;
+        q = slow ? qd : q;
+    }
+    return q;
+#else
+    return div_c(a, d, rd, 1);
+#endif
+}
+FCA_HDI double div_c_wave(double a, double d, double rd) { return div_c(a, d, rd, 1); }
+
 /* |la - lb| / d (physics.h:147-154). */
 template <class R>
 FCA_HD R abs_slope(R la, R lb, R d)

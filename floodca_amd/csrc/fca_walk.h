@@ -396,7 +396,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 out = out + pos_part(-s.qd2_2);
             }
             out = out * dt;
-            s.s1 = limiter_s<R>(out, s.h1, A);
+            s.s1 = limiter_s_wave<R>(out, s.h1, A);
         });
     } else if (qB >= 0 && qB < nr) {
         const int open_row = p.open && (w.edge_cols || qB == 0 || qB == nr - 1);
@@ -556,7 +556,8 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
             {
                 double clamp_here = 0.0;
                 rain = s.pcr * sv.rain_f;
-                hnew = s.h2 + (dt * div_c(net, A, p.rA, p.fA) + rain);
+                hnew = s.h2 + (dt * (LANDF ? div_c_wave(net, A, p.rA) : div_c(net, A, p.rA, p.fA)) +
+                               rain);
                 if (hnew < (R)0) {
                     clamp_here = (double)(-hnew) * (double)A;
                     hnew = (R)0;
