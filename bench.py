@@ -206,6 +206,9 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
     dbase.close()
     same = r_a == r_s == [r_b[i] for i in idx]
     steps = [r[2] for r in r_b]
+    land = [int(np.count_nonzero(base.mask[sc.r0:sc.r0 + sc.side, sc.c0:sc.c0 + sc.side]
+                                 == A.FCA_LAND)) for sc in scen]
+    land_updates = float(sum(n * k for n, k in zip(land, steps)))
     print(f"[bench] ensemble: {n_scen} scenarios ({cells / 1e6:.0f} M cells), base on the "
           f"device in {t_base:.1f}s, strip {strip} ({ {k: round(v, 3) for k, v in tune.items()} }), "
           f"batched {t_b:.2f}s ({ {k: round(v, 2) for k, v in ph.items()} }); "
@@ -218,6 +221,7 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
         "speedup_vs_alone": t_a / t_s,
         "results_identical": bool(same),
         "batched_phases_s": {k: round(v, 3) for k, v in ph.items()},
+        "land_cell_updates_per_s_batched_run": land_updates / ph["run"],
         "scenarios": n_scen, "sample": len(idx), "cells": cells,
         "steps_min_max": [min(steps), max(steps)], "strip_rows": strip,
         "workload": "crops of the 16384^2 DEM, side U[256, 2048], built on the device; "
