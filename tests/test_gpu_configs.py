@@ -2,7 +2,8 @@
 oracle on identical seeded inputs (GPU).
 
 * 512^2 at 90 m, the full 6 h of 50 mm/h uniform rain from a dry start: FP32
-  bit-exact h, hmax, tmax and time-step sequence; FP64 max|dh| <= 1e-12 m.
+  bit-exact h, hmax, tmax and time-step sequence (K=4 and K=8); FP64
+  max|dh| <= 1e-12 m.
   At this width most rows take the fused walk's LAND form, so this is the
   device arithmetic of the fast path checked over a whole flood.
 * 4096^2, 12 steps from a 5 cm warm start (every LAND face carries water
@@ -36,10 +37,11 @@ def _case(size, prec, h0=0.0, T_sim=21600.0, steps=0):
     return a, g, cfg
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_512_full_six_hours(prec):
+@pytest.mark.parametrize("prec,K", [("f32", 4), ("f64", 4), ("f32", 8)])
+def test_512_full_six_hours(prec, K):
     a, g, cfg = _case(512, prec)
-    print(f"case 512^2 6 h {prec}", flush=True)
+    cfg.phys.K = cfg.neigh = K
+    print(f"case 512^2 6 h {prec} K={K}", flush=True)
     go, acc_o, log_o, gh, acc_g, log_h = run_both(a, g, cfg, prec)
     assert acc_o.t == pytest.approx(21600.0)
     assert len(log_o) > 500
