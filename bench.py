@@ -358,7 +358,9 @@ def surrogate_bench(base, args):
                        f"side U{list(pc.side)}, storm D U{list(pc.dur)} h; network "
                        f"{net.n} parameters at {pc.R}^2, BF16 on MFMA, FP32 accumulation; "
                        f"CA reruns of the top {pc.frac_area:.0%} by predicted flooded area "
-                       f"+ {pc.frac_unc:.0%} by uncertainty")
+                       f"+ {pc.frac_unc:.0%} by uncertainty, and of a seeded "
+                       f"{pc.frac_audit:.0%} audit of the unflagged rest (counted in the "
+                       f"time)")
     print(f"[bench] surrogate forward: {ms:.2f} ms per {B} samples, {tflops:.1f} TFLOP/s",
           file=sys.stderr, flush=True)
     return {k: (round(v, 6) if isinstance(v, float) else v) for k, v in res.items()}

@@ -384,6 +384,19 @@ def select(sc: Screen, frac_area=0.02, frac_unc=0.02):
     return top_a, order_u[:ku]
 
 
+def audit_sample(n: int, flagged, frac: float, seed: int) -> np.ndarray:
+    """A seeded uniform sample (count rounded up, sorted) of the members not
+    in flagged: the CA runs them so that the error of the predictions the
+    pipeline delivers without a rerun is measured, not inferred from the
+    validation rows."""
+    rest = np.setdiff1d(np.arange(n), np.asarray(flagged))
+    k = min(len(rest), int(math.ceil(frac * n)))
+    if k == 0:
+        return np.zeros(0, dtype=np.int64)
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(rest, size=k, replace=False))
+
+
 def judged_error(net: nn.Net, terrain: Terrain, base: Grid, scens, R=256, batch=512,
                  strip_rows=0, timing=None, log=None, dbase=None):
     """CA reruns of scens: returns (per-scenario CA results, per-scenario
@@ -433,6 +446,7 @@ class PipelineCfg:
     n_cand: int = 100_000
     frac_area: float = 0.02
     frac_unc: float = 0.02
+    frac_audit: float = 0.01   # unflagged members the CA also runs, to judge the rest
     n_alone: int = 64          # candidates timed one session each (the all-CA baseline)
     epochs: int = 30
     batch: int = 64
@@ -529,26 +543,33 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     sc = screen(net, terrain, cands, pc.R, pc.screen_batch)
     res["t_screen_s"] = sc.seconds
     top_a, top_u = select(sc, pc.frac_area, pc.frac_unc)
-    judged = np.concatenate([top_a, top_u])
+    audit = audit_sample(pc.n_cand, np.concatenate([top_a, top_u]), pc.frac_audit, pc.seed + 3)
+    judged = np.concatenate([top_a, top_u, audit])
     log(f"surrogate: screened {pc.n_cand} candidates in {sc.seconds:.2f}s; CA reruns "
-        f"{len(top_a)} by area + {len(top_u)} by uncertainty")
+        f"{len(top_a)} by area + {len(top_u)} by uncertainty + {len(audit)} unflagged (audit)")
     timing = {}
     t0 = time.perf_counter()
     ca, se, mo = judged_error(net, terrain, base, [cands[i] for i in judged], pc.R,
                               pc.ca_batch, timing=timing,
                               log=lambda m: log(f"surrogate: rerun {m}"), dbase=dbase)
     res["t_rerun_s"] = time.perf_counter() - t0
-    na = len(top_a)
+    na, nf = len(top_a), len(top_a) + len(top_u)
+    dep = depth_error(c_dep, [cands[i] for i in judged], mo)
     res["judged"] = int(len(judged))
+    res["judged_audit"] = int(len(audit))
     res["judged_rmse_m"] = rmse(se)
     res["judged_rmse_area_m"] = rmse(se[:na])
-    res["judged_rmse_uncertainty_m"] = rmse(se[na:])
-    res["judged_rmse_depth_baseline_m"] = rmse(depth_error(c_dep, [cands[i] for i in judged],
-                                                           mo))
+    res["judged_rmse_uncertainty_m"] = rmse(se[na:nf])
+    res["judged_rmse_audit_m"] = rmse(se[nf:]) if len(audit) else float("nan")
+    res["judged_rmse_depth_baseline_m"] = rmse(dep)
+    res["judged_rmse_audit_depth_baseline_m"] = rmse(dep[nf:]) if len(audit) else float("nan")
     res["judged_mean_ca_max_depth_m"] = float(np.mean([r[1] for r in ca]))
     # what the two flagged sets look like against all candidates
-    for key, rows in (("area", top_a), ("uncertainty", top_u), ("candidates", None)):
+    for key, rows in (("area", top_a), ("uncertainty", top_u), ("audit", audit),
+                      ("candidates", None)):
         sel = cands if rows is None else [cands[i] for i in rows]
+        if not len(sel):
+            continue
         res[f"{key}_mean_side"] = float(np.mean([c.side for c in sel]))
         res[f"{key}_mean_storm_depth_m"] = float(np.mean([depth(c) for c in sel]))
     # the per-scenario all-CA baseline on a seeded sample of candidates
@@ -570,10 +591,15 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     res["train_rmse_by_epoch_m"] = trainer.log.train_rmse
     res["val_rmse_by_epoch_m"] = trainer.log.val_rmse
     log(f"surrogate: judged RMSE {res['judged_rmse_m'] * 100:.2f} cm (depth baseline "
-        f"{res['judged_rmse_depth_baseline_m'] * 100:.2f} cm); {res['scenarios_per_h']:.0f} "
+        f"{res['judged_rmse_depth_baseline_m'] * 100:.2f} cm): by area "
+        f"{res['judged_rmse_area_m'] * 100:.2f}, by uncertainty "
+        f"{res['judged_rmse_uncertainty_m'] * 100:.2f}, audit of unflagged "
+        f"{res['judged_rmse_audit_m'] * 100:.2f} cm (depth baseline "
+        f"{res['judged_rmse_audit_depth_baseline_m'] * 100:.2f} cm)")
+    log(f"surrogate: {res['scenarios_per_h']:.0f} "
         f"scenarios/h vs all-CA alone {res['all_ca_alone_scenarios_per_h']:.0f} "
         f"({res['speedup_vs_alone']:.1f}x; {res['speedup_vs_alone_with_training']:.1f}x "
         f"counting dataset and training)")
     res["_objects"] = dict(net=net, data=data, terrain=terrain, screen=sc, cands=cands,
-                           top_a=top_a, top_u=top_u)
+                           top_a=top_a, top_u=top_u, audit=audit)
     return res

@@ -1,7 +1,8 @@
 """The surrogate-screened ensemble end to end at a small size (GPU).
 
 Dataset from the batched CA, training with the library's own kernels,
-screening, CA reruns of the flagged members. What the network gains over the
+screening, CA reruns of the flagged members and of a seeded audit sample of
+the unflagged ones. What the network gains over the
 method without it: on held-out scenarios with CA truth, its hmax maps beat
 the best storm-depth-only predictor (hmax = c x depth on LAND, c fitted on
 the training rows) by at least 10% in RMSE, and its predicted flooded area
@@ -50,8 +51,13 @@ def test_selection_and_reruns(pipeline):
     assert sc.area[top_a].min() >= np.delete(sc.area, top_a).max()
     rest = np.setdiff1d(np.arange(n), np.concatenate([top_a, top_u]))
     assert sc.sigma[top_u].min() >= sc.sigma[rest].max()
-    assert res["judged"] == len(top_a) + len(top_u)
-    assert np.isfinite(res["judged_rmse_m"]) and res["judged_rmse_m"] > 0
+    audit = o["audit"]
+    assert len(audit) == int(np.ceil(0.01 * n)) and np.all(np.diff(audit) > 0)
+    assert not set(audit) & (set(top_a) | set(top_u))
+    assert res["judged"] == len(top_a) + len(top_u) + len(audit)
+    for k in ("judged_rmse_m", "judged_rmse_area_m", "judged_rmse_uncertainty_m",
+              "judged_rmse_audit_m"):
+        assert np.isfinite(res[k]) and res[k] > 0, k
     assert res["scenarios_per_h"] > res["all_ca_alone_scenarios_per_h"]
 
 
