@@ -103,10 +103,21 @@ def build_nn(_=None) -> str:
     return out
 
 
+# FCA_EMU_SANITIZE=1 builds the emulation with AddressSanitizer and UBSan
+# (host code; the walk's indexing checked on the CPU). Run the CPU suite on it
+# with both runtimes preloaded, e.g.
+#   LD_PRELOAD="$(gcc -print-file-name=libasan.so) $(gcc -print-file-name=libubsan.so)"
+#   ASAN_OPTIONS=detect_leaks=0 python -m pytest tests/test_emu.py
+# and build again without the variable afterwards.
+SAN_FLAGS = ["-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+             "-fno-omit-frame-pointer"]
+
+
 def build_emu(prec: str) -> str:
     defs, _ = PRECS[prec]
     out = os.path.join(ROOT, "tests", "emu", f"libwalk_emu_{prec}.so")
-    _run(["g++", *EMU_FLAGS, *defs, os.path.join(ROOT, "tests", "emu", "walk_emu.cpp"),
+    san = SAN_FLAGS if os.environ.get("FCA_EMU_SANITIZE") == "1" else []
+    _run(["g++", *EMU_FLAGS, *san, *defs, os.path.join(ROOT, "tests", "emu", "walk_emu.cpp"),
           "-o", out, "-lm"])
     return out
 
