@@ -19,7 +19,9 @@ the reference serial build's throughput on the host cores (a bounded sample
 of the 512^2 case) as the CPU baseline. Nested in the same line, each with
 its own workload: the 4096^2 grid in FP32 and FP64, the sparse start on 8192^2 (quiet-tile skipping against the
 dense walk), a 96-scenario ensemble (batched against one session per
-scenario) and the surrogate-screened ensemble (4096 CA runs of training data
+scenario), the reference's own ensemble benchmark in shape (100 scenarios of
+1230 x 1206 cells for 24 h each, against its published V100 figure) and the
+surrogate-screened ensemble (4096 CA runs of training data
 and 20000 candidates by default; --sur-cand 100000 is the
 full case).
 """
@@ -230,6 +232,67 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
     }
 
 
+def day_ensemble_bench(prec: str, n_scen: int, n_alone: int, seed: int = 20261210):
+    """The reference's own ensemble benchmark in shape (results/e10_gpu100.csv:2:
+    100 scenarios, one 90 m grid of 1230 x 1206 cells, 24 h each, 609.3
+    scenarios/h on one V100) on synthetic inputs: the same generator's DEM
+    at that size, a uniform event rain of DAY_RDAY x lambda, forcing drawn
+    with the reference generator's distributions. All scenarios in one
+    batched session built on the device, and a seeded sample run one
+    session each (the reference's per-scenario runs), whose results must be
+    identical."""
+    from floodca_amd import ensemble as E
+    a = A.abi(prec)
+    base = synth.make_grid(*E.DAY_GRID, seed=seed, dtype=a.dtype)
+    scen = E.draw_days(n_scen, seed)
+    crops = [E.day_crop(base.shape, s, prec) for s in scen]
+    cfgs = [E.day_config(s, base.dx, prec) for s in scen]
+    rng = np.random.default_rng(seed + 1)
+    idx = sorted(rng.choice(n_scen, size=min(n_alone, n_scen), replace=False).tolist())
+    with E.device_base(base, prec) as dbase:
+        ph = {}
+        t_b, r_b = E.run_crop_batch(dbase, crops, cfgs, prec, phases=ph)
+        t0 = time.perf_counter()
+        r_a = []
+        for i in idx:
+            r_a += E.run_crop_batch(dbase, [crops[i]], [cfgs[i]], prec,
+                                    strip_rows=E.strip_alone(*base.shape))[1]
+        t_a = time.perf_counter() - t0
+    same = r_a == [r_b[i] for i in idx]
+    steps = [r[2] for r in r_b]
+    land = int(np.count_nonzero(base.mask == A.FCA_LAND))
+    # the reference's figure and the median step count of its 100 runs
+    # (results/e10_gpu100.csv:2; results/e10_gpu100/B4.csv, column steps): its
+    # real terrain and rain need ~11x the steps of these synthetic days, so
+    # the comparison is also given per step (same grid size)
+    ref, ref_steps = 609.265, 16317.5
+    sph = n_scen / t_b * 3600.0
+    per_step = sph * float(np.median(steps)) / ref_steps / ref
+    print(f"[bench] reference-shaped ensemble: {n_scen} scenarios of {base.shape[0]}x"
+          f"{base.shape[1]} for 24 h, steps {min(steps)}..{max(steps)}, batched {t_b:.2f}s "
+          f"({sph:.0f} scenarios/h; against the reference's V100 figure {sph / ref:.0f}x, "
+          f"{per_step:.1f}x per step); sample of {len(idx)} alone {t_a:.2f}s, "
+          f"identical={same}", file=sys.stderr, flush=True)
+    return {
+        "scenarios_per_h": sph,
+        "alone_scenarios_per_h": len(idx) / t_a * 3600.0,
+        "speedup_vs_alone": (t_a / len(idx)) / (t_b / n_scen),
+        "results_identical": bool(same),
+        "batched_phases_s": {k: round(v, 3) for k, v in ph.items()},
+        "land_cell_updates_per_s_batched_run": land * float(sum(steps)) / ph["run"],
+        "steps_min_median_max": [min(steps), int(np.median(steps)), max(steps)],
+        "scenarios": n_scen, "sample": len(idx), "land_cells": land,
+        "reference_scenarios_per_h": ref,
+        "reference_median_steps": ref_steps,
+        "reference": "results/e10_gpu100.csv:2, 1x V100, its own grid and daily rain fields",
+        "vs_reference": sph / ref,
+        "vs_reference_per_step": per_step,
+        "workload": f"{n_scen} scenarios on one synthetic {base.shape[0]}x{base.shape[1]} grid "
+                    f"at 90 m, 24 h each, triangular storm of {E.DAY_RDAY} m x lambda "
+                    "(lambda U[0.8, 1.5], D_h in {6, 12, 24} h, phi U[0.25, 0.75]); FP32, K=4",
+    }
+
+
 def sparse_bench(prec: str, size: int = 8192, hours: float = 8.0):
     """Sparse-start flood front (no rain, one inland inflow cell with a
     triangular 500 m^3/s hydrograph peaking at 2 h over 8 h, dry start) on a
@@ -382,6 +445,11 @@ def main() -> int:
                     help="scenarios of the ensemble measured after the main line (0 = skip)")
     ap.add_argument("--ens-alone", type=int, default=24,
                     help="of which this many are also run one session each")
+    ap.add_argument("--day-scenarios", type=int, default=100,
+                    help="scenarios of the reference-shaped ensemble (1230x1206, 24 h each; "
+                         "0 = skip)")
+    ap.add_argument("--day-alone", type=int, default=4,
+                    help="of which this many are also run one session each")
     ap.add_argument("--sparse-size", type=int, default=8192,
                     help="grid side of the sparse-start measurement (0 = skip)")
     ap.add_argument("--surrogate", choices=("off", "on"), default="on",
@@ -490,6 +558,9 @@ def main() -> int:
         line["sparse_start"] = sparse_bench(prec, args.sparse_size)
     if args.ens_scenarios > 0 and prec == "f32" and args.neigh == 4:
         line["ensemble"] = ensemble_bench(g, prec, args.ens_scenarios, args.ens_alone)
+    if args.day_scenarios > 0 and prec == "f32" and args.neigh == 4:
+        line["reference_ensemble"] = day_ensemble_bench(prec, args.day_scenarios,
+                                                        args.day_alone)
     if args.surrogate == "on" and prec == "f32" and args.neigh == 4:
         line["surrogate"] = surrogate_bench(g, args)
     if not args.no_cpu:

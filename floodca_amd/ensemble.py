@@ -120,20 +120,80 @@ def run_batched_crops(dbase, scens, dx: float, prec="f32", strip_rows=0, phases=
                       run_wall=None, t_sim_cap=None):
     """One batch of crop scenarios built on the device from dbase (a
     DeviceBase): returns (seconds, per-scenario (flooded, max depth, steps)),
-    the time covering the scenario table, the run, the reductions and the
-    release, as run_batched."""
+    the time covering the configs, the scenario table, the run, the
+    reductions and the release, as run_batched."""
     t0 = time.perf_counter()
-    res = []
     cfgs = [config_of(s, dx, prec) for s in scens]
     if t_sim_cap is not None:
         cfgs = [_with_tsim(c, min(float(c.T_sim), t_sim_cap)) for c in cfgs]
-    b = Batch.from_crops(dbase, [crop_of(s, prec) for s in scens], cfgs, prec=prec,
-                         strip_rows=strip_rows)
+    crops = [crop_of(s, prec) for s in scens]
+    t_cfg = time.perf_counter() - t0
+    secs, res = run_crop_batch(dbase, crops, cfgs, prec, strip_rows, phases, run_wall)
+    return t_cfg + secs, res
+
+
+# The reference's own ensemble benchmark restated on synthetic inputs: 100
+# scenarios on one 90 m grid of 1230 x 1206 cells, 24 h each, forcing drawn
+# as its scenario generator draws it (results/e10_gpu100.csv:2,
+# prep/floodca_prep/scenarios.py:34-53, src/hybrid/main_ens_mpi_cuda.c:69-120).
+DAY_GRID = (1230, 1206)
+DAY_RDAY = 0.03   # event rain depth (m) before lambda; C = 1
+
+
+@dataclass
+class DayScenario:
+    idx: int
+    lam: float
+    dur_h: int
+    phi: float
+
+
+def draw_days(n: int, seed: int):
+    """Forcing tuples drawn with the distributions of the reference's
+    scenario generator (scenarios.py:44-46): lambda U[0.8, 1.5], storm
+    duration one of 6, 12, 24 h, peak position phi U[0.25, 0.75]."""
+    rng = np.random.default_rng(seed)
+    lam = rng.uniform(0.8, 1.5, size=n)
+    dur = rng.choice(np.array([6, 12, 24]), size=n)
+    phi = rng.uniform(0.25, 0.75, size=n)
+    return [DayScenario(i, float(lam[i]), int(dur[i]), float(phi[i])) for i in range(n)]
+
+
+def day_crop(shape, sc: DayScenario, prec: str = "f32") -> dict:
+    """The whole base grid with the scenario's rain depth DAY_RDAY x lambda
+    and the base roughness."""
+    dt = A.abi(prec).dtype
+    return dict(r0=0, c0=0, nrows=shape[0], ncols=shape[1],
+                rday=float(dt.type(DAY_RDAY * sc.lam)),
+                n_class=[float(v) for v in MANNING_CLASSES.astype(dt)])
+
+
+def day_config(sc: DayScenario, dx: float, prec: str = "f32"):
+    """A 24 h run under the scenario's triangular storm."""
+    a = A.abi(prec)
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, dx)
+    cfg.hyeto = A.FCA_HYETO_TRIANGLE
+    cfg.D_h, cfg.phi = float(sc.dur_h), sc.phi
+    cfg.T_sim = 86400.0
+    cfg.steps = 0
+    return cfg
+
+
+def run_crop_batch(dbase, crops, cfgs, prec="f32", strip_rows=0, phases=None, run_wall=None):
+    """One batch of crops (dicts as crop_of gives) with their configs, built
+    on the device from dbase: returns (seconds, per-scenario (flooded, max
+    depth, steps)), the time covering the scenario table, the run, the
+    reductions and the release; run_wall, if a list, receives the seconds of
+    the run alone; phases, if a dict, the seconds of each phase."""
+    t0 = time.perf_counter()
+    res = []
+    b = Batch.from_crops(dbase, crops, cfgs, prec=prec, strip_rows=strip_rows)
     try:
         t1 = time.perf_counter()
         b.run()
         t2 = time.perf_counter()
-        for i in range(len(scens)):
+        for i in range(len(crops)):
             fl, md = b.flooded(i, THRESHOLDS)
             res.append((fl, md, b.steps(i)))
         t3 = time.perf_counter()

@@ -69,3 +69,40 @@ def test_crop_window_outside_the_base_is_an_error():
     with E.device_base(base) as db:
         with pytest.raises(HipError, match="outside"):
             Batch.from_crops(db, [crop], [E.config_of(sc, base.dx)])
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_day_scenarios_match_the_oracle(prec):
+    """The reference-shaped ensemble's scenarios (bench.py): the whole,
+    non-square base grid as one crop with its own event rain and storm,
+    batched, against the oracle's run of the same grid built on the host
+    (dry start, C = 1, Rday = the crop's depth), over the whole 24 h."""
+    from oracle import Oracle
+    a = A.abi(prec)
+    base = synth.make_grid(70, 53, seed=20261211, dtype=a.dtype)
+    scens = E.draw_days(3, 4)
+    crops = [E.day_crop(base.shape, s, prec) for s in scens]
+    cfgs = [E.day_config(s, base.dx, prec) for s in scens]
+    print(f"case day scenarios {prec}", flush=True)
+    with E.device_base(base, prec) as db:
+        b = Batch.from_crops(db, crops, cfgs, prec=prec)
+        b.run()
+        o = Oracle(prec)
+        for i, (cr, cfg) in enumerate(zip(crops, cfgs)):
+            g = base.copy()
+            g.h[...] = 0
+            g.hmax[...] = 0
+            g.tmax[...] = 0
+            g.C[...] = 1
+            g.Rday[...] = a.dtype.type(cr["rday"])
+            acc_o, _ = o.run(g, cfg)
+            out = g.copy()
+            acc_g = b.read(i, out)
+            assert acc_g.step == acc_o.step == b.steps(i) > 0
+            if prec == "f32":
+                for k in ("h", "hmax", "tmax"):
+                    assert np.array_equal(getattr(g, k), getattr(out, k)), (i, k)
+            else:
+                assert np.abs(g.h - out.h).max() <= 1e-12
+                assert np.abs(g.hmax - out.hmax).max() <= 1e-12
+        b.close()
