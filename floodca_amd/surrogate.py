@@ -465,6 +465,20 @@ class PipelineCfg:
     train_sigma: float = 0.5
 
 
+def rmse_interval(se, n_boot: int = 2000, seed: int = 0, q=(0.025, 0.975)):
+    """Percentile bootstrap interval of rmse(se) over its rows (scenarios
+    resampled with replacement, seeded)."""
+    se = np.asarray(se, dtype=np.float64)
+    if len(se) == 0:
+        return (float("nan"), float("nan"))
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, len(se), size=(n_boot, len(se)))
+    num = se[idx, 0].sum(axis=1)
+    den = np.maximum(se[idx, 1].sum(axis=1), 1e-30)
+    r = np.sqrt(num / den)
+    return (float(np.quantile(r, q[0])), float(np.quantile(r, q[1])))
+
+
 def _spearman(a, b) -> float:
     ra = np.argsort(np.argsort(a)).astype(float)
     rb = np.argsort(np.argsort(b)).astype(float)
@@ -561,6 +575,7 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     res["judged_rmse_area_m"] = rmse(se[:na])
     res["judged_rmse_uncertainty_m"] = rmse(se[na:nf])
     res["judged_rmse_audit_m"] = rmse(se[nf:]) if len(audit) else float("nan")
+    res["judged_rmse_audit_95ci_m"] = list(rmse_interval(se[nf:], seed=pc.seed))
     res["judged_rmse_depth_baseline_m"] = rmse(dep)
     res["judged_rmse_audit_depth_baseline_m"] = rmse(dep[nf:]) if len(audit) else float("nan")
     res["judged_mean_ca_max_depth_m"] = float(np.mean([r[1] for r in ca]))
@@ -594,7 +609,9 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
         f"{res['judged_rmse_depth_baseline_m'] * 100:.2f} cm): by area "
         f"{res['judged_rmse_area_m'] * 100:.2f}, by uncertainty "
         f"{res['judged_rmse_uncertainty_m'] * 100:.2f}, audit of unflagged "
-        f"{res['judged_rmse_audit_m'] * 100:.2f} cm (depth baseline "
+        f"{res['judged_rmse_audit_m'] * 100:.2f} cm (95% interval "
+        f"{res['judged_rmse_audit_95ci_m'][0] * 100:.2f}-"
+        f"{res['judged_rmse_audit_95ci_m'][1] * 100:.2f}; depth baseline "
         f"{res['judged_rmse_audit_depth_baseline_m'] * 100:.2f} cm)")
     log(f"surrogate: {res['scenarios_per_h']:.0f} "
         f"scenarios/h vs all-CA alone {res['all_ca_alone_scenarios_per_h']:.0f} "

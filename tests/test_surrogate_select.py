@@ -52,3 +52,16 @@ def test_audit_sample_edge_cases():
     assert len(SG.audit_sample(10, [0, 1], 0.0, 1)) == 0
     assert sorted(SG.audit_sample(5, [0, 1, 2], 1.0, 1).tolist()) == [3, 4]
     assert len(SG.audit_sample(4, [0, 1, 2, 3], 0.5, 1)) == 0
+
+
+def test_rmse_interval_brackets_the_rmse_and_narrows_with_more_rows():
+    rng = np.random.default_rng(5)
+    w = rng.uniform(100, 1000, 400)
+    se = np.stack([w * rng.gamma(2.0, 0.0008, 400), w], 1)   # per-row (sum sq err, weight)
+    r = SG.rmse(se)
+    lo, hi = SG.rmse_interval(se, seed=1)
+    assert lo < r < hi
+    lo4, hi4 = SG.rmse_interval(np.tile(se, (4, 1)), seed=1)
+    assert hi4 - lo4 < 0.7 * (hi - lo)       # ~1/2 for 4x the rows
+    assert SG.rmse_interval(se, seed=1) == (lo, hi)
+    assert all(np.isnan(SG.rmse_interval(np.zeros((0, 2)))))
