@@ -26,6 +26,34 @@ def test_draw_is_seeded_and_in_range():
     assert abs(np.median(f) - 1.0) < 0.05 and abs(np.std(np.log(f)) - 0.5) < 0.03
 
 
+def test_day_draw_follows_the_reference_generator_distributions():
+    """draw_days uses the distributions of the reference's scenario generator
+    (prep/floodca_prep/scenarios.py:44-46): lambda U[0.8, 1.5], D_h one of
+    6, 12, 24 h, phi U[0.25, 0.75]; seeded."""
+    a = E.draw_days(4000, 7)
+    assert [(s.lam, s.dur_h, s.phi) for s in a] == \
+           [(s.lam, s.dur_h, s.phi) for s in E.draw_days(4000, 7)]
+    lam = np.array([s.lam for s in a])
+    phi = np.array([s.phi for s in a])
+    dur = np.array([s.dur_h for s in a])
+    assert lam.min() >= 0.8 and lam.max() < 1.5 and abs(lam.mean() - 1.15) < 0.01
+    assert phi.min() >= 0.25 and phi.max() < 0.75 and abs(phi.mean() - 0.5) < 0.01
+    assert set(dur.tolist()) == {6, 12, 24}
+    assert all(abs(np.mean(dur == d) - 1 / 3) < 0.03 for d in (6, 12, 24))
+
+
+def test_day_crop_and_config():
+    sc = E.DayScenario(0, 1.25, 12, 0.4)
+    cr = E.day_crop(E.DAY_GRID, sc)
+    assert (cr["r0"], cr["c0"], cr["nrows"], cr["ncols"]) == (0, 0) + E.DAY_GRID
+    assert cr["rday"] == float(np.float32(E.DAY_RDAY * 1.25))
+    assert cr["n_class"] == [float(v) for v in synth.MANNING_CLASSES.astype(np.float32)]
+    cfg = E.day_config(sc, 90.0)
+    assert cfg.hyeto == A.FCA_HYETO_TRIANGLE and cfg.D_h == 12.0
+    assert cfg.phi == pytest.approx(0.4) and cfg.T_sim == 86400.0 and cfg.steps == 0
+    assert cfg.phys.dx == 90.0 and cfg.phys.A == 8100.0
+
+
 def test_build_crops_scales_roughness_and_sets_the_storm(base):
     sc = E.Scenario(idx=0, r0=10, c0=20, side=100, factor=1.5, dur_h=4.0,
                     nscale=np.linspace(0.8, 1.2, 8))
