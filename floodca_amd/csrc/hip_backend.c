@@ -825,6 +825,12 @@ static int sim_run_hooked(FcaHipSim *s, const RunHooks *hk)
                 n = cfg->steps - c.step;
             }
         }
+        if (hk && hk->poll && hk->poll_every > 0 && n > hk->poll_every - c.step % hk->poll_every) {
+            /* end the batch at the next multiple of poll_every, so that poll
+             * runs after every poll_every steps as the reference's loop
+             * calls it (kernels.cu:967-968) */
+            n = hk->poll_every - c.step % hk->poll_every;
+        }
         if (fca_hip_sim_advance(s, n) < 0 || read_ctrl(s, &c) != 0) {
             return -2;
         }

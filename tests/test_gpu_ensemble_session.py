@@ -65,3 +65,32 @@ def test_ensemble_session_matches_the_oracle_scenario_by_scenario(prec):
     assert np.array_equal(c32.astype(np.uint64), total)
     assert np.array_equal(c16.astype(np.uint64), np.minimum(total, 65535))
     assert total.max() >= 2
+
+
+def test_poll_runs_after_every_poll_every_steps():
+    """fca_hip_ens_run calls poll(pctx) after every poll_every steps, as the
+    reference's loop does (kernels.cu:967-968), and polling leaves the
+    results unchanged."""
+    a = A.abi("f32")
+    base = synth.make_grid(64, 77, seed=33, dtype=a.dtype, rday=3.0)
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, base.dx)
+    cfg.T_sim = 5400.0
+    o = Oracle("f32")
+    print("case ensemble poll", flush=True)
+    ens = Ensemble(base, cfg, THR)
+    try:
+        g1 = base.copy()
+        acc1 = o.acc_init(g1, cfg)
+        r1 = ens.run(g1, cfg, acc1)[:2]
+        seen = []
+        g2 = base.copy()
+        acc2 = o.acc_init(g2, cfg)
+        r2 = ens.run(g2, cfg, acc2, poll=lambda ctx: seen.append(1), poll_every=7)[:2]
+    finally:
+        ens.close()
+    assert acc1.step == acc2.step > 20
+    assert len(seen) == acc2.step // 7
+    assert r1 == r2
+    for f in ("t", "V", "V_rain", "V_sea", "V_bnd", "cap_total"):
+        assert getattr(acc1, f) == getattr(acc2, f), f
