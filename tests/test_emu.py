@@ -113,3 +113,29 @@ def test_quiet_tiles_skip_without_changing_results(prec, K, bnd):
     assert [r.dt for r in log_o] == [r.dt for r in log_e] == [r.dt for r in log_n]
     assert acc_o.cap_active == acc_e.cap_active
     assert acc_e.V_rain == acc_n.V_rain and acc_e.V_bnd == acc_n.V_bnd
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("days_present", [3, 2])
+def test_forcing_days_match_oracle(prec, days_present, tmp_path):
+    """Day-boundary Rday reloads through the emulated controller (the pause,
+    the reload scaled by lambda, cr rebuilt) and a missing day file, which
+    ends the run at T_sim."""
+    a, g, cfg = make_case(29, 23, prec=prec, K=4, T_sim=86400.0 * 3, seed=13)
+    cfg.hyeto = A.FCA_HYETO_UNIFORM
+    cfg.lambda_ = 0.8
+    rng = np.random.default_rng(1)
+    for d in range(days_present):
+        (rng.random(g.shape) * (0.3 + d)).astype(np.float32).tofile(tmp_path / f"r{d}.bin")
+    cfg.n_force_days = 3
+    for d in range(3):
+        cfg.force_files[d].value = str(tmp_path / f"r{d}.bin").encode()
+    o = Oracle(prec)
+    go, ge = g.copy(), g.copy()
+    acc_o, log_o = o.run(go, cfg, log_cap=100000)
+    acc_e, log_e = emu_run(ge, cfg, o.acc_init(ge, cfg), prec, strip_rows=7)
+    for k in ("h", "hmax", "tmax"):
+        assert np.array_equal(getattr(go, k), getattr(ge, k)), k
+    assert [r.dt for r in log_o] == [r.dt for r in log_e]
+    assert acc_o.step == acc_e.step and acc_o.t == acc_e.t == 86400.0 * 3
+    assert acc_o.force_day == acc_e.force_day == days_present - 1
