@@ -6,7 +6,8 @@ accumulators, CSV report) and linked against libfloodca_hip*.so -- the
 executable a maintainer adds beside floodca_cuda. Run on a grid dir written
 for the reference, it must leave the hmax and h_final rasters the oracle
 computes from the same dir (FP32 bit for bit, FP64 within 1e-12 m), and
-append a CSV row with the reference serial driver's columns. Built by
+append a CSV row with the reference serial driver's columns and write the
+reference's run manifest. Built by
 __graft_entry__.build() where the reference tree exists; skipped otherwise.
 """
 import os
@@ -61,6 +62,12 @@ def test_reference_driver_runs_the_hip_backend(prec, tmp_path):
         assert np.array_equal(hmax, go.hmax) and np.array_equal(h, go.h)
     else:
         assert np.abs(hmax - go.hmax).max() <= 1e-12 and np.abs(h - go.h).max() <= 1e-12
+    # the run manifest the reference's report writes (cli.c:124-151)
+    import json
+    man = json.loads((out_dir / "run_manifest.json").read_text())
+    assert man["backend"] == "hip" and man["steps"] == acc.step and man["t"] == 3600.0
+    assert abs(man["E_mass"]) < 1e-5 and "gfx" in man["gpu"]
+    assert man["V_rain"] == pytest.approx(acc.V_rain, rel=1e-6)
     # the CSV row has the reference serial driver's columns
     csv_ref = tmp_path / "serial.csv"
     subprocess.run([_exe(f"floodca_serial_{prec}"), "--grid", str(grid_dir), "--steps", "2",
