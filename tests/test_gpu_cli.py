@@ -77,3 +77,43 @@ def test_reference_driver_runs_the_hip_backend(prec, tmp_path):
     assert ours[0] == theirs[0]
     assert len(ours[1].split(",")) == len(theirs[1].split(","))
     assert ",hip," in ours[1]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_reference_driver_with_a_forcing_csv(prec, tmp_path):
+    """--forcing through the reference's CSV loader (rain.c:120-200) and the
+    library's day reloads: the same rasters as the oracle with the same day
+    files."""
+    exe = _exe(f"floodca_hip_{prec}")
+    grid_dir, out_dir = tmp_path / "grid", tmp_path / "out"
+    g = synth.make_grid(41, 53, seed=23)
+    write_dir(str(grid_dir), g)
+    rng = np.random.default_rng(3)
+    for d in range(2):
+        (rng.random(g.shape) * (0.05 + 0.1 * d)).astype(np.float32).tofile(tmp_path / f"r{d}.bin")
+    (tmp_path / "forcing.csv").write_text(
+        "day_index,date,file\n0,2021-07-01,r0.bin\n1,2021-07-02,r1.bin\n")
+    print(f"case floodca_hip forcing {prec}", flush=True)
+    r = subprocess.run([exe, "--grid", str(grid_dir), "--forcing", str(tmp_path / "forcing.csv"),
+                        "--tsim", str(86400.0 * 2), "--hyeto", "uniform", "--out", str(out_dir)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    a = A.abi(prec)
+    go = load_dir(str(grid_dir), dtype=a.dtype)
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, go.dx)
+    cfg.hyeto = A.FCA_HYETO_UNIFORM
+    cfg.T_sim = 86400.0 * 2
+    cfg.n_force_days = 2
+    for d in range(2):
+        cfg.force_files[d].value = f"{tmp_path}/r{d}.bin".encode()
+    acc, _ = Oracle(prec).run(go, cfg)
+    assert acc.force_day == 1
+    done = [ln for ln in r.stdout.splitlines() if ln.startswith("done backend=hip")]
+    assert done and f"steps={acc.step} " in done[0], r.stdout[-2000:]
+    hmax = read_bin(str(out_dir / "hmax.bin"))
+    h = read_bin(str(out_dir / "h_final.bin"))
+    if prec == "f32":
+        assert np.array_equal(hmax, go.hmax) and np.array_equal(h, go.h)
+    else:
+        assert np.abs(hmax - go.hmax).max() <= 1e-12 and np.abs(h - go.h).max() <= 1e-12
