@@ -117,3 +117,49 @@ def test_reference_driver_with_a_forcing_csv(prec, tmp_path):
         assert np.array_equal(hmax, go.hmax) and np.array_equal(h, go.h)
     else:
         assert np.abs(hmax - go.hmax).max() <= 1e-12 and np.abs(h - go.h).max() <= 1e-12
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_reference_driver_warm_start_rain_lambda_k8(prec, tmp_path):
+    """--warmstart (h and hmax from a float32 raster, cli.c:97-112), --rain
+    with --lambda (grid.c:256-268), --neigh 8, --U 2, a closed boundary, a
+    triangular storm and output events, all through the reference's CLI code,
+    against the oracle on the same inputs."""
+    exe = _exe(f"floodca_hip_{prec}")
+    grid_dir, out_dir = tmp_path / "grid", tmp_path / "out"
+    g = synth.make_grid(59, 71, seed=29)
+    write_dir(str(grid_dir), g)
+    rng = np.random.default_rng(4)
+    h0 = np.where(g.mask == A.FCA_LAND, rng.random(g.shape) * 0.2, 0).astype(np.float32)
+    h0.tofile(tmp_path / "warm.bin")
+    rain = (rng.random(g.shape) * 0.4).astype(np.float32)
+    rain.tofile(tmp_path / "rain.bin")
+    args = ["--grid", str(grid_dir), "--warmstart", str(tmp_path / "warm.bin"),
+            "--rain", str(tmp_path / "rain.bin"), "--lambda", "1.3", "--neigh", "8", "--U", "2",
+            "--boundary", "closed", "--hyeto", "triangle", "--D_h", "1.5", "--phi", "0.3",
+            "--out_every", "900", "--tsim", "5400", "--out", str(out_dir)]
+    print(f"case floodca_hip warm start {prec}", flush=True)
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    a = A.abi(prec)
+    go = load_dir(str(grid_dir), dtype=a.dtype)
+    go.h[...] = h0.astype(a.dtype)
+    go.hmax[...] = go.h
+    go.Rday[...] = rain.astype(a.dtype) * a.dtype.type(1.3)
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, go.dx)
+    cfg.phys.K = cfg.neigh = 8
+    cfg.phys.U = 2
+    cfg.phys.boundary = A.FCA_BND_CLOSED
+    cfg.hyeto, cfg.D_h, cfg.phi = A.FCA_HYETO_TRIANGLE, 1.5, 0.3
+    cfg.out_every, cfg.T_sim = 900.0, 5400.0
+    cfg.lambda_ = 1.3
+    acc, _ = Oracle(prec).run(go, cfg)
+    done = [ln for ln in r.stdout.splitlines() if ln.startswith("done backend=hip")]
+    assert done and f"steps={acc.step} " in done[0], r.stdout[-2000:]
+    hmax = read_bin(str(out_dir / "hmax.bin"))
+    h = read_bin(str(out_dir / "h_final.bin"))
+    if prec == "f32":
+        assert np.array_equal(hmax, go.hmax) and np.array_equal(h, go.h)
+    else:
+        assert np.abs(hmax - go.hmax).max() <= 1e-12 and np.abs(h - go.h).max() <= 1e-12
