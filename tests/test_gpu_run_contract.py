@@ -10,7 +10,6 @@ visible devices)."""
 import numpy as np
 import pytest
 
-from floodca_amd import abi as A
 from floodca_amd.hip import backend
 from oracle import Oracle
 

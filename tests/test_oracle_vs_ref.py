@@ -3,7 +3,6 @@
 the forcing time line, on per-cell time steps, and on whole runs including a
 two-day forcing CSV. Skipped where the reference tree was not available at
 build time (the golden fixtures in tests/golden carry its results there)."""
-import os
 
 import numpy as np
 import pytest
