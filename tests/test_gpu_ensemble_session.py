@@ -94,3 +94,45 @@ def test_poll_runs_after_every_poll_every_steps():
     assert r1 == r2
     for f in ("t", "V", "V_rain", "V_sea", "V_bnd", "cap_total"):
         assert getattr(acc1, f) == getattr(acc2, f), f
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_ensemble_session_reloads_forcing_days(prec, tmp_path):
+    """A scenario with day files (model.c:338-358) run in the session: the day
+    reloads happen per scenario and the result is the oracle's; a second
+    scenario without forcing on the same session is unaffected by them."""
+    a = A.abi(prec)
+    base = synth.make_grid(48, 61, seed=35, dtype=a.dtype)
+    cfg0 = A.default_cfg(a)
+    A.set_dx(a, cfg0, base.dx)
+    rng = np.random.default_rng(6)
+    for d in range(2):
+        (rng.random(base.shape) * (0.05 + 0.1 * d)).astype(np.float32).tofile(tmp_path / f"r{d}.bin")
+    o = Oracle(prec)
+    print(f"case ensemble forcing {prec}", flush=True)
+    ens = Ensemble(base, cfg0, THR, prec=prec)
+    try:
+        for forced in (True, False):
+            cfg = A.default_cfg(a)
+            A.set_dx(a, cfg, base.dx)
+            cfg.hyeto, cfg.T_sim = A.FCA_HYETO_UNIFORM, 86400.0 * 2 if forced else 7200.0
+            if forced:
+                cfg.lambda_ = 1.1
+                cfg.n_force_days = 2
+                for d in range(2):
+                    cfg.force_files[d].value = str(tmp_path / f"r{d}.bin").encode()
+            g, go = base.copy(), base.copy()
+            acc_o, _ = o.run(go, cfg)
+            acc = o.acc_init(g, cfg)
+            fl, md, _ = ens.run(g, cfg, acc)
+            assert acc.step == acc_o.step and acc.t == acc_o.t
+            assert acc.force_day == acc_o.force_day == (1 if forced else -1)
+            for f in ("V", "V_rain", "V_sea", "V_bnd", "clamp_err"):
+                ref = getattr(acc_o, f)
+                assert getattr(acc, f) == pytest.approx(ref, rel=1e-9, abs=1e-9), f
+            counts = np.zeros(len(THR) * base.z.size, dtype=np.uint32)
+            fl_o, md_o = o.ens_count(go, THR, counts)
+            assert fl == fl_o
+            assert md == pytest.approx(md_o, abs=1e-12 if prec == "f64" else 0)
+    finally:
+        ens.close()
