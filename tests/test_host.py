@@ -101,3 +101,13 @@ def test_library_rejects_empty_grid():
     acc = a.FcaAcc()
     assert not b.lib.fca_hip_sim_open(C.byref(g), C.byref(cfg), C.byref(acc), None)
     assert "empty grid" in b.error()
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """No CPU fallback: without the built HIP library the binding raises
+    instead of computing anything."""
+    from floodca_amd import hip
+
+    monkeypatch.setenv("FCA_HIP_LIB", str(tmp_path / "absent.so"))
+    with pytest.raises(hip.HipError, match="not built"):
+        hip.Backend("f32")
