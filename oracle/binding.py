@@ -38,8 +38,10 @@ class _Lib:
 class Oracle(_Lib):
     """The C restatement of the step (oracle/fca_oracle.c)."""
 
-    def __init__(self, prec: str = "f32"):
-        super().__init__(os.path.join(ORACLE_DIR, f"liboracle_{prec}.so"), prec)
+    def __init__(self, prec: str = "f32", path: str | None = None):
+        """path: another build of fca_oracle.c (the tolerance study's variants);
+        default the oracle itself."""
+        super().__init__(path or os.path.join(ORACLE_DIR, f"liboracle_{prec}.so"), prec)
         a, L, r = self.a, self.lib, self.a.real
         L.or_face_flux.restype = r
         L.or_face_flux.argtypes = _face_sig(a)

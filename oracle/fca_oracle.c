@@ -36,11 +36,22 @@
 
 /* ---- face and cell formulas (src/kernels/physics.h) --------------------- */
 
+/* The float-mode libm detours. A build for the tolerance study
+ * (oracle/tolerance_study.py) may replace them with approximations through
+ * these macros; the oracle's own build keeps the reference's expressions. */
+#ifndef ORACLE_CBRT_SQ
+#define ORACLE_CBRT_SQ(hf, sq) ((real)cbrt(sq))
+#endif
+#ifndef ORACLE_SQRT_SLOPE
+#define ORACLE_SQRT_SLOPE(x) ((real)sqrt((double)(x)))
+#endif
+
 /* physics.h:19-24: hf^(2/3) through a double cbrt of the exact double hf^2. */
 real or_cbrt_sq(real hf)
 {
     double sq = (double)hf * (double)hf;
-    return (real)cbrt(sq);
+    (void)hf;
+    return ORACLE_CBRT_SQ(hf, sq);
 }
 
 /* physics.h:51-125. Positive result = flow from cell a to cell b. */
@@ -95,7 +106,7 @@ real or_face_flux(real za, real ha, real na, uint8_t ma, real zb, real hb, real 
     }
     /* Manning velocity, capped at the critical velocity (physics.h:105-112) */
     slope = ((dL < (real)0) ? -dL : dL) / d;
-    v = ((real)1 / nf) * or_cbrt_sq(hf) * (real)sqrt((double)slope);
+    v = ((real)1 / nf) * or_cbrt_sq(hf) * ORACLE_SQRT_SLOPE(slope);
     vcr = (real)sqrt((double)p->g * (double)hf);
     if (v > vcr) {
         v = vcr;
@@ -200,7 +211,7 @@ real or_cell_dt(const real *z, const real *h, const real *n, const uint8_t *mask
     if (smax < (real)0) {
         smax = (real)0;
     }
-    v = ((real)1 / nn) * or_cbrt_sq(hh) * (real)sqrt((double)smax);
+    v = ((real)1 / nn) * or_cbrt_sq(hh) * ORACLE_SQRT_SLOPE(smax);
     vcr = (real)sqrt((double)p->g * (double)hh);
     if (v > vcr) {
         v = vcr;
