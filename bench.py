@@ -8,9 +8,11 @@ uniform hyetograph), dry start, K = 4, U = 1, FP32, open boundary. T_sim is
 set to 1e9 s so the run is bounded by the step count, as the reference's
 benchmarks do (--steps N --tsim 1e9).
 
-A "step" is one launch of the fused step kernel over the whole grid. W warmup
-steps run first (untimed; they also wet every LAND cell), then exactly K steps
-are timed with HIP events on the session's stream, inputs resident in HBM.
+A "step" is one launch of the fused step kernel over the whole grid. The
+flood first spins up for a fixed 300 steps (untimed; they wet every LAND cell,
+so the timed window is the same mid-flood state whatever W and K are), then W
+warmup steps run (untimed), then exactly K steps are timed with HIP events on
+the session's stream, inputs resident in HBM.
 
 Printed: one JSON line with value = LAND cell-updates/s (the reference's MCUPS
 definition x 1e6, src/common/log.c:133-135), the HBM roofline of the step
@@ -24,6 +26,11 @@ scenario), the reference's own ensemble benchmark in shape (100 scenarios of
 surrogate-screened ensemble (4096 CA runs of training data
 and 20000 candidates by default; --sur-cand 100000 is the
 full case).
+
+``--calibrate-counters`` instead copies a buffer of known size (hipMemcpy
+device to device) so that rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of that
+dispatch can be set against the bytes it must move; the factors scale the
+fused kernel's counters into HBM bytes (profiles/).
 """
 from __future__ import annotations
 
@@ -91,6 +98,26 @@ class HipTimer:
         self.rt.hipDeviceSynchronize()
 
 
+def calibrate_counters(mib: int = 1024, reps: int = 3) -> int:
+    """Copies mib MiB device to device reps times (each copy reads and writes
+    exactly mib MiB), for the counter calibration under rocprofv3 --pmc."""
+    rt = HipTimer().rt
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if rt.hipMalloc(C.byref(a), C.c_size_t(n)) or rt.hipMalloc(C.byref(b), C.c_size_t(n)):
+        print("hipMalloc failed", file=sys.stderr)
+        return 1
+    rt.hipMemset(a, 1, C.c_size_t(n))
+    rt.hipDeviceSynchronize()
+    for _ in range(reps):
+        rt.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    rt.hipDeviceSynchronize()
+    rt.hipFree(a)
+    rt.hipFree(b)
+    print(f"copied {mib} MiB x {reps}", flush=True)
+    return 0
+
+
 def load_traffic(workload: str):
     """Per-launch HBM bytes measured with rocprofv3 --pmc (FETCH_SIZE x2 +
     WRITE_SIZE, scaled by what the counters read for copies of known size)
@@ -102,6 +129,21 @@ def load_traffic(workload: str):
     except (OSError, ValueError):
         return None
     return None if t is None else float(t["bytes"])
+
+
+VALU_ISSUE_PER_S = 256 * 2.4e9  # wave64 VALU instructions per second: one per CU per clock
+
+
+def load_valu(workload: str):
+    """wave64 VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU) for
+    this workload, as committed under profiles/ (None if not measured)."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p, encoding="utf-8") as fp:
+            t = json.load(fp).get(workload)
+    except (OSError, ValueError):
+        return None
+    return None if t is None or "valu_insts" not in t else float(t["valu_insts"])
 
 
 def cpu_baseline(seconds: float, prec: str):
@@ -146,9 +188,13 @@ def cpu_baseline(seconds: float, prec: str):
     out = {"value": n_land * steps / wall, "unit": "cell-updates/s", "cores": 1, "kind": kind,
            "sample": f"512^2 grid, {n_land} LAND cells, {steps} steps from a dry start with "
                      f"50 mm/h rain, FP32, serial, wall {wall:.1f} s"}
-    omp = cpu_baseline_omp(seconds, prec)
-    if omp:
-        out["omp"] = omp
+    try:
+        out["omp"] = cpu_baseline_omp(seconds, prec)
+    except (OSError, subprocess.SubprocessError, AttributeError, ValueError) as e:
+        out["omp"] = {"error": f"{type(e).__name__}: {e}"}
+    if kind != "reference":
+        out["note"] = (f"{exe} is not built (oracle/build_oracle.sh needs the reference "
+                       "sources): the oracle restatement was timed instead")
     return out
 
 
@@ -437,7 +483,9 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--spinup", type=int, default=300,
+                    help="untimed steps of the flood before the warmup (the timed state)")
     ap.add_argument("--size", type=int, default=16384)
     ap.add_argument("--prec", choices=("f32", "f64"), default="f32")
     ap.add_argument("--kernel", choices=("fused", "split"), default="fused")
@@ -445,7 +493,7 @@ def main() -> int:
     ap.add_argument("--neigh", type=int, default=4, choices=(4, 8))
     ap.add_argument("--no-land-form", action="store_true",
                     help="fused walk in its general form only (same results)")
-    ap.add_argument("--ens-scenarios", type=int, default=96,
+    ap.add_argument("--ens-scenarios", type=int, default=512,
                     help="scenarios of the ensemble measured after the main line (0 = skip)")
     ap.add_argument("--ens-alone", type=int, default=24,
                     help="of which this many are also run one session each")
@@ -460,8 +508,8 @@ def main() -> int:
                     help="run the surrogate-screened ensemble after the main line")
     ap.add_argument("--sur-train", type=int, default=4096,
                     help="CA runs of its dataset")
-    ap.add_argument("--sur-cand", type=int, default=20000,
-                    help="candidates it screens (the full case: 100000)")
+    ap.add_argument("--sur-cand", type=int, default=100000,
+                    help="candidates it screens")
     ap.add_argument("--sur-epochs", type=int, default=30)
     ap.add_argument("--sur-train-sigma", type=float, default=0.5,
                     help="spread of the training storms' log intensity")
@@ -471,7 +519,11 @@ def main() -> int:
                     help="skip the 4096^2 FP32/FP64 measurement")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-counters", action="store_true",
+                    help="only copy 1 GiB device to device 3 times (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_counters:
+        return calibrate_counters()
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -501,7 +553,7 @@ def main() -> int:
     flags = A.FCA_HIP_NO_LAND_FORM if args.no_land_form else 0
     with Sim(g, cfg, acc, prec=prec, kernel=kern, strip_rows=args.strip, flags=flags) as sim:
         stream = C.c_void_p(sim.stream)
-        sim.advance(args.warmup)
+        sim.advance(args.spinup + args.warmup)
         sim.sync()
         timer.device_sync()
         timer.start(stream)
@@ -510,8 +562,9 @@ def main() -> int:
         timer.device_sync()
         out = sim.read()
     steps_done = out.step
-    if steps_done != args.warmup + args.steps:
-        print(f"[bench] warning: {steps_done} steps taken of {args.warmup + args.steps}",
+    first = args.spinup + args.warmup + 1
+    if steps_done != first - 1 + args.steps:
+        print(f"[bench] warning: {steps_done} steps taken of {first - 1 + args.steps}",
               file=sys.stderr)
     ms_per_step = ms / args.steps
     value = n_land * args.steps / (ms * 1e-3)
@@ -519,8 +572,9 @@ def main() -> int:
     achieved = bps / (ms_per_step * 1e-3) / 1e9
     workload = f"fused-{prec}-K{args.neigh}-{args.size}"
     # measured for the default kernel options only
-    traffic = (load_traffic(workload) if kern == A.FCA_HIP_FUSED and args.strip == 0
-               and not args.no_land_form else None)
+    default_kernel = kern == A.FCA_HIP_FUSED and args.strip == 0 and not args.no_land_form
+    traffic = load_traffic(workload) if default_kernel else None
+    valu = load_valu(workload) if default_kernel else None
     emass = abs(out.V + out.V_sea + out.V_bnd - out.V_rain - out.V0 - out.clamp_err) / \
         max(out.V_rain + out.V0, 1e-30)
     print(f"[bench] {ms_per_step:.4f} ms/step, {achieved:.0f} GB/s algorithmic, "
@@ -542,7 +596,9 @@ def main() -> int:
         "config": {
             "workload": f"plain WCA step, single grid {args.size}^2 at 90 m, {prec.upper()}, "
                         f"K={args.neigh}, U=1, 50 mm/h uniform rain from a dry start, "
-                        f"{args.kernel} kernel",
+                        f"{args.kernel} kernel; timed steps {first}-{first - 1 + args.steps} "
+                        f"of the flood",
+            "timed_steps": [first, first - 1 + args.steps],
             "grid": [args.size, args.size], "land_cells": n_land, "all_cells": n_all,
             "kernel": args.kernel, "neigh": args.neigh, "sim_time_s": out.t,
             "all_cell_updates_per_s": n_all * args.steps / (ms * 1e-3),
@@ -554,6 +610,12 @@ def main() -> int:
             "traffic": traffic,
             "bytes_per_launch": bps,
             "bytes_rule": "per LAND cell z,n,cr,h,hmax + mask read, h written; 1 B mask per other cell",
+            # the instruction-issue ceiling beside the HBM one: wave64 VALU
+            # instructions per launch (rocprofv3, profiles/) at one per CU per clock
+            "issue": (None if valu is None else {
+                "valu_insts_per_launch": valu, "issue_ms": valu / VALU_ISSUE_PER_S * 1e3,
+                "frac": valu / VALU_ISSUE_PER_S / (ms_per_step * 1e-3),
+                "rate_per_s": VALU_ISSUE_PER_S}),
         },
     }
     if args.grid4096 and args.size != 4096 and prec == "f32" and args.neigh == 4:

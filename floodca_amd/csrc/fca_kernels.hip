@@ -233,13 +233,6 @@ struct FusedArgs {
     int skip;
 };
 
-/* Optional occupancy floor for the fused kernel (waves per SIMD). */
-#ifdef FCA_FUSED_MIN_WAVES
-#define FCA_FUSED_ATTR __attribute__((amdgpu_waves_per_eu(FCA_FUSED_MIN_WAVES, 8)))
-#else
-#define FCA_FUSED_ATTR
-#endif
-
 /* One workgroup of the fused step: wg of nwg workgroups walking a.ntiles
  * tiles of one grid. */
 template <int K8, int QT>
@@ -335,7 +328,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
 }
 
 template <int K8, int QT>
-__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR k_fused(FusedArgs a)
+__global__ void __launch_bounds__(FCA_BLOCK) k_fused(FusedArgs a)
 {
     fused_body<K8, QT>(a, blockIdx.x, gridDim.x);
 }
@@ -355,7 +348,7 @@ struct ScenDev {
 };
 
 template <int K8>
-__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR
+__global__ void __launch_bounds__(FCA_BLOCK)
     k_fused_batch(const ScenDev *scen, const int2 *wgs, long long k)
 {
     const int2 e = wgs[blockIdx.x];
@@ -369,6 +362,36 @@ __global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_ATTR
     a.io.rec = S->rec;
     a.io.rec_cap = S->rec_cap;
     fused_body<K8, 0>(a, e.y, S->nblocks);
+}
+
+/* Adds a block's balance terms to the block's own slot (a block always
+ * covers the same cells, so each slot sums the same cells every step, in a
+ * fixed order: the totals repeat bit for bit from run to run). */
+__device__ void block_add_slot(FcaAccSlot *slot, double vr, double cl, double vs, double vb,
+                               long long cap)
+{
+    __shared__ double sh[FCA_BLOCK / 64][4];
+    __shared__ long long shc[FCA_BLOCK / 64];
+    const int wave = threadIdx.x >> 6;
+    vr = wave_sum(vr), cl = wave_sum(cl), vs = wave_sum(vs), vb = wave_sum(vb);
+    cap = wave_sum(cap);
+    if ((threadIdx.x & 63) == 0) {
+        sh[wave][0] = vr, sh[wave][1] = cl, sh[wave][2] = vs, sh[wave][3] = vb;
+        shc[wave] = cap;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[4] = {0, 0, 0, 0};
+        long long c = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+            for (int j = 0; j < 4; j++) {
+                t[j] += sh[w][j];
+            }
+            c += shc[w];
+        }
+        slot->v_rain += t[0], slot->clamp += t[1], slot->v_sea += t[2], slot->v_bnd += t[3];
+        slot->cap += c;
+    }
 }
 
 /* ---- split step ----------------------------------------------------------- */
@@ -437,10 +460,7 @@ __global__ void __launch_bounds__(FCA_BLOCK) k_split_faces(SplitArgs a)
             }
         }
     }
-    cap_n = wave_sum(cap_n);
-    if ((threadIdx.x & 63) == 0 && cap_n) {
-        atomicAdd((unsigned long long *)&a.acc->cap, (unsigned long long)cap_n);
-    }
+    block_add_slot(a.acc + blockIdx.x, 0.0, 0.0, 0.0, 0.0, cap_n);
 }
 
 /* Outgoing volume and limiter per cell (model.c:180-238, 488-502). */
@@ -581,13 +601,7 @@ __global__ void __launch_bounds__(FCA_BLOCK) k_split_update(SplitArgs a)
             }
         }
     }
-    vr = wave_sum(vr), cl = wave_sum(cl), vs = wave_sum(vs), vb = wave_sum(vb);
-    if ((threadIdx.x & 63) == 0) {
-        if (vr != 0.0) atomicAdd(&a.acc->v_rain, vr);
-        if (cl != 0.0) atomicAdd(&a.acc->clamp, cl);
-        if (vs != 0.0) atomicAdd(&a.acc->v_sea, vs);
-        if (vb != 0.0) atomicAdd(&a.acc->v_bnd, vb);
-    }
+    block_add_slot(a.acc + blockIdx.x, vr, cl, vs, vb, 0);
 }
 
 /* Next step's candidates from the updated state (model.c:111-156). */
@@ -712,6 +726,16 @@ __global__ void __launch_bounds__(FCA_BLOCK)
             t += sh[w];
         }
         red[blockIdx.x] = t;
+    }
+}
+
+/* Fills n 32-bit words with v (used where a caller might capture the work
+ * into a graph, which must not hold memset nodes). */
+__global__ void __launch_bounds__(FCA_BLOCK) k_fill_u32(uint32_t *p, size_t n, uint32_t v)
+{
+    for (size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * FCA_BLOCK) {
+        p[i] = v;
     }
 }
 
@@ -1010,4 +1034,16 @@ extern "C" int fca_k_ens_count(const FcaDev *d, uint32_t *counts, int nthr,
     hipLaunchKernelGGL(k_ens_count, dim3(g), dim3(FCA_BLOCK), 0, st, d->hmax, d->mask, N, t,
                        counts, d->red);
     return hipGetLastError() == hipSuccess ? (int)g : -1;
+}
+
+extern "C" int fca_k_zero(void *p, size_t bytes, hipStream_t st)
+{
+    const size_t n = (bytes + 3) / 4;
+    unsigned g = blocks_for(n);
+    if (n == 0) {
+        return 0;
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(g < 4096 ? g : 4096), dim3(FCA_BLOCK), 0, st,
+                       (uint32_t *)p, n, 0u);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -49,6 +49,9 @@ typedef struct FcaDev {
     void *arena;            /* one allocation holding every buffer above */
 } FcaDev;
 
+/* Zeroes bytes (rounded up to a multiple of 4) at p with a fill kernel. */
+int fca_k_zero(void *p, size_t bytes, hipStream_t st);
+
 /* Device table of n grids for the batched fused step (NULL on failure); free
  * with hipFree. */
 void *fca_k_batch_table(const FcaDev *const *devs, int n);

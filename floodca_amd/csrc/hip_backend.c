@@ -168,10 +168,16 @@ void fca_hip_sim_close(FcaHipSim *s)
     free(s);
 }
 
+/* Faces between cells (K=4: east and south; K=8 adds two diagonals). */
+static long count_inner_faces(long nr, long nc, const FcaPhys *p)
+{
+    return nr * (nc - 1) + (nr - 1) * nc + (p->K == 8 ? 2 * (nr - 1) * (nc - 1) : 0);
+}
+
 static long count_faces(const FcaGrid *g, const FcaPhys *p)
 {
     long nr = g->nrows, nc = g->ncols, f, r, c;
-    f = nr * (nc - 1) + (nr - 1) * nc + (p->K == 8 ? 2 * (nr - 1) * (nc - 1) : 0);
+    f = count_inner_faces(nr, nc, p);
     if (p->boundary == FCA_BND_OPEN) {
         for (r = 0; r < nr; r++) {
             for (c = 0; c < nc; c++) {
@@ -249,28 +255,21 @@ static void set_div_flags(FcaDev *d, int n_ok, int no_land_form)
 #endif
 }
 
-static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
-                     const FcaHipOpts *o, hipStream_t shared)
+/* Default strip height: about three tiles per resident wavefront slot
+ * (4096 at this kernel's occupancy), so no launch ends in a long tail,
+ * between 16 rows and 96 (where the 5 halo rows a strip re-walks cost 5%).
+ * A 16384^2 grid gets 96 rows, 4096^2 gets 23 (measured at 4096^2: 0.295
+ * ms/step at 16 or 24 rows, 0.418 at 96). */
+static int default_strip_rows(int nrows, int ncols)
 {
-    FcaDev *d = &s->d;
-    const size_t N = (size_t)g->nrows * (size_t)g->ncols, B = N * sizeof(real);
-    d->nr = g->nrows;
-    d->nc = g->ncols;
-    d->K = cfg->phys.K;
-    d->kernel = o ? o->kernel : FCA_HIP_FUSED;
-    /* Default strip height: about three tiles per resident wavefront slot
-     * (4096 at this kernel's occupancy), so no launch ends in a long tail,
-     * between 16 rows and 96 (where the 5 halo rows a strip re-walks cost
-     * 5%). A 16384^2 grid gets 96 rows, 4096^2 gets 23 (measured at 4096^2:
-     * 0.295 ms/step at 16 or 24 rows, 0.418 at 96). */
-    if (o && o->strip_rows > 0) {
-        d->strip_rows = o->strip_rows;
-    } else {
-        const long groups = (g->ncols + FCA_TILE_COLS - 1) / FCA_TILE_COLS;
-        long rows = (long)g->nrows * groups / (3 * 4096);
-        d->strip_rows = (int)(rows < 16 ? 16 : (rows > 96 ? 96 : rows));
-    }
-    d->rec_cap = (o && o->record_steps > 0) ? o->record_steps : 0;
+    const long groups = (ncols + FCA_TILE_COLS - 1) / FCA_TILE_COLS;
+    const long rows = (long)nrows * groups / (3 * 4096);
+    return (int)(rows < 16 ? 16 : (rows > 96 ? 96 : rows));
+}
+
+/* The run's physical constants and the controller's parameters. */
+static void set_run_consts(FcaDev *d, const FcaRunCfg *cfg)
+{
     d->phys = cfg->phys;
     d->ctl.T_sim = (double)cfg->T_sim;
     d->ctl.out_every = (double)cfg->out_every;
@@ -280,6 +279,21 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
     d->ctl.steps = cfg->steps;
     d->ctl.U = cfg->phys.U;
     d->ctl.n_force_days = cfg->n_force_days;
+}
+
+static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
+                     const FcaHipOpts *o, hipStream_t shared)
+{
+    FcaDev *d = &s->d;
+    const size_t N = (size_t)g->nrows * (size_t)g->ncols, B = N * sizeof(real);
+    d->nr = g->nrows;
+    d->nc = g->ncols;
+    d->K = cfg->phys.K;
+    d->kernel = o ? o->kernel : FCA_HIP_FUSED;
+    d->strip_rows = (o && o->strip_rows > 0) ? o->strip_rows
+                                             : default_strip_rows(g->nrows, g->ncols);
+    d->rec_cap = (o && o->record_steps > 0) ? o->record_steps : 0;
+    set_run_consts(d, cfg);
     s->no_land_form = o && (o->flags & FCA_HIP_NO_LAND_FORM);
     set_div_flags(d, 0, s->no_land_form);
     d->max_blocks = (o && o->max_blocks > 0) ? o->max_blocks : 0;
@@ -295,7 +309,9 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
         HIP_OR(hipStreamCreate(&s->st), -2);
         s->own_stream = 1;
     }
-    d->nslots = d->kernel == FCA_HIP_FUSED ? d->nblocks : 1;
+    /* one balance slot per workgroup: the fused walk's, or one per block of
+     * the split kernels' cell-parallel launches */
+    d->nslots = d->kernel == FCA_HIP_FUSED ? d->nblocks : (int)((N + 255) / 256);
     d->nred = NRED * 5;
     {
         ArenaReq req[24];
@@ -399,10 +415,10 @@ static int sim_start(FcaHipSim *s, const FcaAcc *acc)
     c.state = FCA_CTL_RUN;
     c.hbuf = 0;
     c.dt_base = (double)acc->dt_base;
-    HIP_OR(hipMemcpy(d->ctrl, &c, sizeof(c), hipMemcpyHostToDevice), -2);
-    HIP_OR(hipMemset(d->slots, 0, (size_t)d->nslots * sizeof(FcaAccSlot)), -2);
-    if (d->tflags) { /* no tile is known quiet before its first walk */
-        HIP_OR(hipMemset(d->tflags, 0, (size_t)2 * d->ntiles), -2);
+    HIP_OR(hipMemcpyAsync(d->ctrl, &c, sizeof(c), hipMemcpyHostToDevice, s->st), -2);
+    if (fca_k_zero(d->slots, (size_t)d->nslots * sizeof(FcaAccSlot), s->st) != 0 ||
+        (d->tflags && fca_k_zero(d->tflags, (size_t)2 * d->ntiles, s->st) != 0)) {
+        return fail(-2, "clearing the slots failed"); /* no tile is known quiet yet */
     }
     if (d->tflags && s->skip_wanted) {
         /* skipping can only help where a LAND tile gets no rain or the
@@ -859,31 +875,28 @@ int fca_hip_sim_run(FcaHipSim *s)
     return sim_run_hooked(s, NULL);
 }
 
-/* Sums the per-workgroup balance slots in a fixed order. */
+/* Sums the per-workgroup balance slots in a fixed order (read through a
+ * buffer on the stack: nothing is allocated). */
 static int read_slots(FcaHipSim *s, double sums[4], long long *cap)
 {
     FcaDev *d = &s->d;
-    FcaAccSlot *h;
-    int i;
-    h = (FcaAccSlot *)malloc((size_t)d->nslots * sizeof(FcaAccSlot));
-    if (!h) {
-        return fail(-2, "out of host memory");
-    }
-    if (hipMemcpy(h, d->slots, (size_t)d->nslots * sizeof(FcaAccSlot), hipMemcpyDeviceToHost) !=
-        hipSuccess) {
-        free(h);
-        return fail(-2, "slot read-back failed");
-    }
+    FcaAccSlot buf[1024];
+    int i0, i, n;
     sums[0] = sums[1] = sums[2] = sums[3] = 0.0;
     *cap = 0;
-    for (i = 0; i < d->nslots; i++) {
-        sums[0] += h[i].v_rain;
-        sums[1] += h[i].clamp;
-        sums[2] += h[i].v_sea;
-        sums[3] += h[i].v_bnd;
-        *cap += h[i].cap;
+    for (i0 = 0; i0 < d->nslots; i0 += n) {
+        n = d->nslots - i0 < 1024 ? d->nslots - i0 : 1024;
+        HIP_OR(hipMemcpyAsync(buf, d->slots + i0, (size_t)n * sizeof(FcaAccSlot),
+                              hipMemcpyDeviceToHost, s->st), -2);
+        HIP_OR(hipStreamSynchronize(s->st), -2);
+        for (i = 0; i < n; i++) {
+            sums[0] += buf[i].v_rain;
+            sums[1] += buf[i].clamp;
+            sums[2] += buf[i].v_sea;
+            sums[3] += buf[i].v_bnd;
+            *cap += buf[i].cap;
+        }
     }
-    free(h);
     return 0;
 }
 
@@ -1672,6 +1685,50 @@ int fca_hip_batch_flooded(FcaHipBatch *b, int i, int nthr, const double *thr, lo
         *max_depth = m > *max_depth ? m : *max_depth;
     }
     return 0;
+}
+
+int fca_hip_batch_add_counts_u32(FcaHipBatch *b, int nthr, const double *thr, uint32_t *host)
+{
+    uint32_t *dev = NULL, *tmp = NULL;
+    size_t N, n, j;
+    int i, rc = 0;
+    if (!b || nthr < 1 || nthr > 4 || !thr || !host) {
+        return fail(-1, "bad arguments");
+    }
+    N = b->sims[0]->N;
+    for (i = 1; i < b->n; i++) {
+        if (b->sims[i]->d.nr != b->sims[0]->d.nr || b->sims[i]->d.nc != b->sims[0]->d.nc) {
+            return fail(-1, "count rasters need scenarios of one shape (scenario %d differs)", i);
+        }
+    }
+    n = (size_t)nthr * N;
+    HIP_OR(hipMalloc((void **)&dev, n * sizeof(uint32_t)), -2);
+    tmp = (uint32_t *)malloc(n * sizeof(uint32_t));
+    if (!tmp) {
+        hipFree(dev);
+        return fail(-2, "out of host memory");
+    }
+    if (fca_k_zero(dev, n * sizeof(uint32_t), b->st) != 0) {
+        rc = fail(-2, "count clear failed");
+    }
+    /* counts[j*N + a] += (LAND && hmax >= thr[j]) over every scenario, on the
+     * device; the order of the additions does not matter (integers) */
+    for (i = 0; i < b->n && rc == 0; i++) {
+        if (fca_k_ens_count(&b->sims[i]->d, dev, nthr, thr, b->st) < 0) {
+            rc = fail(-2, "count launch failed");
+        }
+    }
+    if (rc == 0 && (hipMemcpyAsync(tmp, dev, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                   b->st) != hipSuccess ||
+                    hipStreamSynchronize(b->st) != hipSuccess)) {
+        rc = fail(-2, "count read-back failed");
+    }
+    for (j = 0; j < n && rc == 0; j++) {
+        host[j] += tmp[j];
+    }
+    free(tmp);
+    hipFree(dev);
+    return rc;
 }
 
 int fca_hip_batch_status(FcaHipBatch *b, int i, int *step, double *t)

@@ -159,6 +159,31 @@ def draw_days(n: int, seed: int):
     return [DayScenario(i, float(lam[i]), int(dur[i]), float(phi[i])) for i in range(n)]
 
 
+SCENARIO_FIELDS = ("scenario_id", "base_date", "lambda", "D_h", "phi", "seed")
+
+
+def write_scenario_csv(path: str, dates, n: int, seed: int) -> list[dict]:
+    """A scenario table in the reference's columns (scenarios.py:18): base
+    date drawn from `dates`, lambda U[0.8, 1.5], D_h one of 6, 12, 24 h, phi
+    U[0.25, 0.75] and a per-scenario seed, with lambda and phi written with 17
+    significant digits as the reference's generator writes them
+    (scenarios.py:34-53). Returns the rows."""
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, len(dates), size=n)
+    lam = rng.uniform(0.8, 1.5, size=n)
+    dur = rng.choice(np.array([6, 12, 24]), size=n)
+    phi = rng.uniform(0.25, 0.75, size=n)
+    seeds = rng.integers(0, 2**31 - 1, size=n, dtype=np.int64)
+    rows = [{"scenario_id": k, "base_date": dates[int(idx[k])], "lambda": f"{float(lam[k]):.17g}",
+             "D_h": int(dur[k]), "phi": f"{float(phi[k]):.17g}", "seed": int(seeds[k])}
+            for k in range(n)]
+    with open(path, "w", encoding="utf-8") as fp:
+        fp.write(",".join(SCENARIO_FIELDS) + "\n")
+        for r in rows:
+            fp.write(",".join(str(r[f]) for f in SCENARIO_FIELDS) + "\n")
+    return rows
+
+
 def day_crop(shape, sc: DayScenario, prec: str = "f32") -> dict:
     """The whole base grid with the scenario's rain depth DAY_RDAY x lambda
     and the base roughness."""

@@ -315,6 +315,11 @@ int fca_hip_batch_steplog(FcaHipBatch *b, int i, FcaHipStepRec *out, int cap);
  * LAND hmax, as the ensemble reports them (ens_common.c:166-188). */
 int fca_hip_batch_flooded(FcaHipBatch *b, int i, int nthr, const double *thr, long *flooded,
                           double *max_depth);
+/* Adds, for every scenario of the batch (all of one shape, nthr <= 4), the
+ * LAND cells with hmax >= thr[j] into host[j*N + a] (uint32, threshold-major
+ * as fca_hip_ens_counts_u32), so the batches of one ensemble accumulate into
+ * one raster; the count rasters of src/hybrid/ens_common.c:166-221. */
+int fca_hip_batch_add_counts_u32(FcaHipBatch *b, int nthr, const double *thr, uint32_t *host);
 /* Scenario i's steps taken and simulated time. */
 int fca_hip_batch_status(FcaHipBatch *b, int i, int *step, double *t);
 /* Seconds spent in fca_hip_batch_run so far. */

@@ -1,10 +1,11 @@
 #!/usr/bin/env bash
-# Builds floodca_hip_{f32,f64} into oracle/_ref/: the reference's own
-# command-line driver code (src/common, compiled where it lies) around
-# floodca_amd/csrc/main_hip.c, linked against this repository's
-# libfloodca_hip*.so -- the executable a maintainer would add beside the
-# reference's floodca_cuda. Used by the tests to run the library behind the
-# reference's CLI, grid loading and CSV report. Needs the reference tree
+# Builds floodca_hip_{f32,f64} and floodca_ens_hip_{f32,f64} into oracle/_ref/:
+# the reference's own command-line driver code (src/common, compiled where it
+# lies) around floodca_amd/csrc/main_hip.c and main_ens_hip.c, linked against
+# this repository's libfloodca_hip*.so -- the executables a maintainer would
+# add beside the reference's floodca_cuda and floodca_ens_mpi_cuda. Used by
+# the tests to run the library behind the reference's CLI, grid loading, rain
+# files and CSV report. Needs the reference tree
 # (REF, default /root/reference) and the built libraries; otherwise does
 # nothing.
 set -euo pipefail
@@ -30,4 +31,7 @@ for prec in f32 f64; do
     "$CC" "${FLAGS[@]}" "${def[@]}" -w "${INC[@]}" "$ROOT/floodca_amd/csrc/main_hip.c" \
         "${SRC[@]}" -L"$ROOT/floodca_amd" -l:"$lib" -Wl,-rpath,'$ORIGIN/../../floodca_amd' -lm \
         -o "$HERE/_ref/floodca_hip_${prec}"
+    "$CC" "${FLAGS[@]}" "${def[@]}" -w "${INC[@]}" -I"$ROOT/include" \
+        "$ROOT/floodca_amd/csrc/main_ens_hip.c" "${SRC[@]}" -L"$ROOT/floodca_amd" -l:"$lib" \
+        -Wl,-rpath,'$ORIGIN/../../floodca_amd' -lm -o "$HERE/_ref/floodca_ens_hip_${prec}"
 done

@@ -1,11 +1,18 @@
-"""Crop scenarios built on the device against the same scenarios built on the
-host (GPU): a batch opened from a resident base grid (fca_hip_batch_open_crops)
-ends every scenario bit for bit where a batch of host-built crop grids
-(fca_hip_batch_open) ends it -- h, hmax, tmax, the accumulators, the step
-count and the flooded counts -- and a crop run alone ends where it ends in a
-batch. The host-built grids are themselves checked against the oracle by the
-batch tests, so this pins the device build of a scenario (window, roughness
-per class, storm depth, dry start, open-face count)."""
+"""Crop scenarios (GPU).
+
+* Config 3's own scenarios against the oracle: the ensemble generator's draw
+  (side, storm duration, lognormal storm factor, per-class Manning scale)
+  built on the host for the oracle and on the device for a batch, triangle
+  storms run to T_sim = D; FP32 bit for bit, FP64 within 1e-12 m.
+* Device-built against host-built: a batch opened from a resident base grid
+  (fca_hip_batch_open_crops) ends every scenario bit for bit where a batch of
+  host-built crop grids (fca_hip_batch_open) ends it -- h, hmax, tmax, the
+  accumulators, the step count and the flooded counts -- and a crop run alone
+  ends where it ends in a batch. This pins the device build of a scenario
+  (window, roughness per class, storm depth, dry start, open-face count).
+"""
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 
@@ -105,4 +112,74 @@ def test_day_scenarios_match_the_oracle(prec):
             else:
                 assert np.abs(g.h - out.h).max() <= 1e-12
                 assert np.abs(g.hmax - out.hmax).max() <= 1e-12
+        b.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_config3_scenarios_match_the_oracle(prec):
+    """The batched ensemble's scenarios (E.draw with the config's
+    distributions: storm duration U[2, 12] h with both ends taken, storm
+    factor lognormal(0, 0.5), each Manning class x U[0.8, 1.2]; sides
+    U[256, 300] so the oracle finishes in seconds) on crops of a 1024^2 base of
+    the benchmark generator, one in its west corner (all SEA) and one in its
+    east corner:
+    the oracle runs E.build's host grid of each to T_sim = D, the GPU runs
+    the device-built crops in one batch."""
+    from oracle import Oracle
+
+    a = A.abi(prec)
+    base = synth.make_grid(1024, 1024, seed=20261213, dtype=a.dtype)
+    n = 5 if prec == "f32" else 3
+    scens = E.draw(n, 31, base.shape, side=(256, 300), dur=(2.0, 12.0))
+    scens[0].dur_h = 2.0
+    scens[1].dur_h = 12.0
+    scens[0].r0, scens[0].c0 = 0, 0
+    scens[-1].r0 = base.shape[0] - scens[-1].side
+    scens[-1].c0 = base.shape[1] - scens[-1].side
+    assert min(s.factor for s in scens) < 1.0 < max(s.factor for s in scens)
+    assert all(np.any(np.abs(s.nscale - 1.0) > 0.05) for s in scens)
+    o = Oracle(prec)
+    cases = [E.build(base, s, prec) for s in scens]
+
+    def oracle_run(case):
+        g, cfg = case
+        go = g.copy()
+        acc, log = o.run(go, cfg, log_cap=1 << 16)
+        return go, acc, log
+
+    print(f"case config-3 scenarios vs oracle {prec}: sides {[s.side for s in scens]}, "
+          f"D {[round(s.dur_h, 2) for s in scens]} h", flush=True)
+    with ThreadPoolExecutor(len(cases)) as ex:  # the oracle releases the GIL
+        refs = list(ex.map(oracle_run, cases))
+    with E.device_base(base, prec) as db:
+        b = Batch.from_crops(db, [E.crop_of(s, prec) for s in scens],
+                             [E.config_of(s, base.dx, prec) for s in scens], prec=prec,
+                             record_steps=1 << 16)
+        b.run()
+        for i, ((g, cfg), (go, acc_o, log_o)) in enumerate(zip(cases, refs)):
+            out = g.copy()
+            acc_g = b.read(i, out)
+            log_g = b.steplog(i, 1 << 16)
+            assert acc_o.t == pytest.approx(scens[i].dur_h * 3600.0)
+            assert acc_g.step == acc_o.step == len(log_o) > 100, i
+            if prec == "f32":
+                for k in ("h", "hmax", "tmax"):
+                    assert np.array_equal(getattr(go, k), getattr(out, k)), (i, k)
+                assert [r.dt for r in log_g] == [r.dt for r in log_o], i
+                assert [r.dt_base for r in log_g] == [r.dt_base for r in log_o], i
+                for k in ("cap_active", "cap_total", "dt_min_hits"):
+                    assert getattr(acc_g, k) == getattr(acc_o, k), (i, k)
+            else:
+                for k in ("h", "hmax"):
+                    assert np.abs(getattr(go, k) - getattr(out, k)).max() <= 1e-12, (i, k)
+            for k in ("V", "V_rain", "V_sea", "V_bnd", "clamp_err"):
+                assert getattr(acc_g, k) == pytest.approx(getattr(acc_o, k), rel=1e-9,
+                                                          abs=1e-6), (i, k)
+            fl, md = b.flooded(i, E.THRESHOLDS)
+            land = go.mask == A.FCA_LAND
+            assert fl == [int(np.count_nonzero(land & (go.hmax >= t))) for t in E.THRESHOLDS]
+            # the largest LAND hmax, at least 0 (ens_common.c:170-178); the
+            # corner crop of the base is all SEA
+            md_ref = max(0.0, float(go.hmax[land].max())) if land.any() else 0.0
+            assert md == pytest.approx(md_ref, rel=1e-12 if prec == "f32" else 1e-9)
         b.close()
