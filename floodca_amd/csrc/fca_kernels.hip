@@ -18,6 +18,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -739,6 +740,73 @@ __global__ void __launch_bounds__(FCA_BLOCK) k_fill_u32(uint32_t *p, size_t n, u
     }
 }
 
+/* Host-side checks of fca_hip_run for fields that live in device memory:
+ * out[0] becomes nonzero if some LAND roughness lies outside [1e-12, 2^100]
+ * (fca_land_n_ok), out[1] counts the open-boundary faces of the LAND cells
+ * on the domain edge (the open part of the reference's cap_total count). */
+__global__ void __launch_bounds__(FCA_BLOCK)
+    k_grid_scan(const R *n, const uint8_t *mask, int nr, int nc, int open,
+                unsigned long long *out)
+{
+    const size_t N = (size_t)nr * nc;
+    unsigned long long bad = 0, faces = 0;
+    for (size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * FCA_BLOCK) {
+        if (mask[i] != FCA_LAND) {
+            continue;
+        }
+        bad |= !(n[i] >= (R)1e-12 && n[i] <= (R)0x1p100);
+        if (open) {
+            const int r = (int)(i / (size_t)nc), c = (int)(i - (size_t)r * nc);
+            faces += (c == 0) + (c == nc - 1) + (r == 0) + (r == nr - 1);
+        }
+    }
+    bad = wave_sum(bad);
+    faces = wave_sum(faces);
+    if ((threadIdx.x & 63) == 0) {
+        if (bad) {
+            atomicAdd(&out[0], bad);
+        }
+        if (faces) {
+            atomicAdd(&out[1], faces);
+        }
+    }
+}
+
+/* ---- workspace of fca_hip_run_noalloc ---------------------------------------
+ * One array per device, reserved when the library's code object is loaded and
+ * held until the process ends. Host code never passes `scratch` itself to a
+ * kernel (on the host the name is only a stand-in for the symbol): its device
+ * address comes from hipGetSymbolAddress, once per device, with that device
+ * current. Every call on a device uses the same array, so calls that overlap
+ * on one device race; each call clears what it needs cleared. */
+static __device__ __align__(256) unsigned char scratch[FCA_NOALLOC_SCRATCH_BYTES];
+
+#define FCA_MAX_DEVICES 64
+static void *g_scratch_ptr[FCA_MAX_DEVICES];
+static pthread_mutex_t g_scratch_mu = PTHREAD_MUTEX_INITIALIZER;
+
+extern "C" int fca_k_noalloc_scratch(void **ptr, size_t *bytes)
+{
+    int dev = -1, rc = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= FCA_MAX_DEVICES) {
+        return -1;
+    }
+    pthread_mutex_lock(&g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = NULL;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            rc = -1;
+        } else {
+            g_scratch_ptr[dev] = p;
+        }
+    }
+    *ptr = g_scratch_ptr[dev];
+    pthread_mutex_unlock(&g_scratch_mu);
+    *bytes = sizeof(scratch);
+    return rc;
+}
+
 #define FCA_MAXTHR 4
 struct Thr {
     double v[FCA_MAXTHR];
@@ -1045,5 +1113,17 @@ extern "C" int fca_k_zero(void *p, size_t bytes, hipStream_t st)
     }
     hipLaunchKernelGGL(k_fill_u32, dim3(g < 4096 ? g : 4096), dim3(FCA_BLOCK), 0, st,
                        (uint32_t *)p, n, 0u);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fca_k_grid_scan(const FcaDev *d, unsigned long long *out, hipStream_t st)
+{
+    const size_t N = (size_t)d->nr * d->nc;
+    unsigned g = blocks_for(N);
+    if (fca_k_zero(out, 2 * sizeof(unsigned long long), st) != 0) {
+        return -1;
+    }
+    hipLaunchKernelGGL(k_grid_scan, dim3(g < 4096 ? g : 4096), dim3(FCA_BLOCK), 0, st, d->n,
+                       d->mask, d->nr, d->nc, d->phys.boundary == FCA_BND_OPEN, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -49,8 +49,24 @@ typedef struct FcaDev {
     void *arena;            /* one allocation holding every buffer above */
 } FcaDev;
 
+/* Workspace of fca_hip_run_noalloc for grids of up to
+ * FCA_HIP_NOALLOC_MAX_CELLS cells: the second depth buffer and cr (one real per
+ * cell each), plus room for the controller records, candidate keys, balance
+ * slots, tile flags and reduction partials of the worst tile count such a
+ * grid can have (strips of at least 16 rows: at most 2^18 tiles, 15 B each). */
+#define FCA_NOALLOC_SMALL_BYTES ((size_t)4 << 20)
+#define FCA_NOALLOC_SCRATCH_BYTES \
+    ((size_t)2 * FCA_HIP_NOALLOC_MAX_CELLS * sizeof(real) + FCA_NOALLOC_SMALL_BYTES)
+/* Partial sums of the volume reduction in the no-alloc path. */
+#define FCA_NOALLOC_NRED 4096
+/* Device address and size of this device's workspace (hipGetSymbolAddress
+ * once per device). Returns 0, or -1 if the address cannot be had. */
+int fca_k_noalloc_scratch(void **ptr, size_t *bytes);
 /* Zeroes bytes (rounded up to a multiple of 4) at p with a fill kernel. */
 int fca_k_zero(void *p, size_t bytes, hipStream_t st);
+/* out[0] = nonzero if a LAND roughness leaves [1e-12, 2^100]; out[1] = open
+ * faces of LAND edge cells (zeroed first). */
+int fca_k_grid_scan(const FcaDev *d, unsigned long long *out, hipStream_t st);
 
 /* Device table of n grids for the batched fused step (NULL on failure); free
  * with hipFree. */

@@ -68,6 +68,7 @@ struct FcaHipSim {
     int no_land_form;      /* FCA_HIP_NO_LAND_FORM was asked for */
     int own_stream;        /* st was created for this session */
     int skip_wanted;       /* quiet-tile skipping not switched off */
+    int noalloc;           /* fca_hip_run_noalloc: no allocation of any kind */
     hipGraphExec_t gexec;  /* glen step launches, captured once */
     int glen;
 };
@@ -423,9 +424,10 @@ static int sim_start(FcaHipSim *s, const FcaAcc *acc)
     if (d->tflags && s->skip_wanted) {
         /* skipping can only help where a LAND tile gets no rain or the
          * hyetograph can fall to zero; otherwise the kernel without the
-         * quiet-tile bookkeeping runs */
+         * quiet-tile bookkeeping runs (the no-alloc path does not look for
+         * rain-free tiles: either kernel gives the same results) */
         int can = s->cfg->hyeto != FCA_HYETO_UNIFORM || s->cfg->n_force_days > 0;
-        if (!can) {
+        if (!can && !s->noalloc) {
             uint8_t *tn = (uint8_t *)malloc((size_t)d->ntiles);
             int t;
             if (!tn) {
@@ -1027,6 +1029,145 @@ int fca_hip_run(FcaGrid *g, FcaRunCfg *cfg, FcaAcc *acc, FcaTimers *tm)
         }
     }
     fca_hip_sim_close(s);
+    return rc;
+}
+
+/* fca_hip_run on caller-owned device fields with no allocation: the
+ * session lives on this stack frame, its device buffers are carved from the
+ * library's per-device workspace (fca_k_noalloc_scratch), controller
+ * read-backs land in stack variables, and the steps are launched one by one
+ * (instantiating a graph would allocate). */
+int fca_hip_run_noalloc(FcaGrid *g, FcaRunCfg *cfg, FcaAcc *acc, FcaTimers *tm, void *stream)
+{
+    FcaHipSim s;
+    FcaDev *d = &s.d;
+    FcaDevCtrl hctrl[2], c;
+    double hred[FCA_NOALLOC_NRED];
+    unsigned long long scan[2], *dscan;
+    char *base = NULL;
+    size_t bytes = 0, off = 0, N, B;
+    double t0;
+    int rc;
+
+    g_err[0] = 0;
+    if (!g || !cfg || !acc) {
+        return fail(-1, "NULL grid, cfg or acc");
+    }
+    if (check_cfg(g, cfg, NULL) != 0) {
+        return -1;
+    }
+    N = (size_t)g->nrows * (size_t)g->ncols;
+    if (N > (size_t)FCA_HIP_NOALLOC_MAX_CELLS) {
+        return fail(-1, "grid of %d x %d cells exceeds the %d cells fca_hip_run_noalloc accepts",
+                    g->nrows, g->ncols, FCA_HIP_NOALLOC_MAX_CELLS);
+    }
+    if (!g->z || !g->h || !g->n || !g->C || !g->mask || !g->Rday || !g->hmax || !g->tmax) {
+        return fail(-1, "a field of the grid is NULL");
+    }
+    if (cfg->n_force_days > 0) {
+        return fail(-1, "multi-day forcing needs host staging; fca_hip_run_noalloc rejects it");
+    }
+    if (!cfg->reduce_impl[0]) {
+        snprintf(cfg->reduce_impl, sizeof(cfg->reduce_impl), "wave");
+    }
+    t0 = now_s();
+    if (fca_k_noalloc_scratch((void **)&base, &bytes) != 0 || !base) {
+        return fail(-2, "workspace address unavailable: %s", hipGetErrorString(hipGetLastError()));
+    }
+    memset(&s, 0, sizeof(s));
+    s.noalloc = 1;
+    s.cfg = cfg;
+    s.h_ctrl = hctrl;
+    s.h_red = hred;
+    s.st = (hipStream_t)stream;
+    s.N = N;
+    s.batch = 30;
+    s.glen = 0;
+    s.skip_wanted = 1;
+    B = N * sizeof(real);
+    d->nr = g->nrows;
+    d->nc = g->ncols;
+    d->K = cfg->phys.K;
+    d->kernel = FCA_HIP_FUSED;
+    d->strip_rows = default_strip_rows(g->nrows, g->ncols);
+    set_run_consts(d, cfg);
+    fca_k_geometry(d);
+    d->nslots = d->nblocks;
+    d->nred = FCA_NOALLOC_NRED;
+#define FCA_CARVE(p_, n_) ((p_) = (void *)(base + off), off += ((size_t)(n_) + 255) & ~(size_t)255)
+    {
+        void *p;
+        FCA_CARVE(p, B);
+        d->h[1] = (real *)p;
+        FCA_CARVE(p, B);
+        d->cr = (real *)p;
+        FCA_CARVE(p, 2 * sizeof(FcaDevCtrl));
+        d->ctrl = (FcaDevCtrl *)p;
+        FCA_CARVE(d->cand, 3 * sizeof(double));
+        FCA_CARVE(p, (size_t)d->nslots * sizeof(FcaAccSlot));
+        d->slots = (FcaAccSlot *)p;
+        FCA_CARVE(p, sizeof(FcaHipStepRec));
+        d->rec = (FcaHipStepRec *)p;
+        FCA_CARVE(p, (size_t)d->nred * sizeof(double));
+        d->red = (double *)p;
+        FCA_CARVE(p, (size_t)2 * d->ntiles);
+        d->tflags = (uint8_t *)p;
+        FCA_CARVE(p, (size_t)d->ntiles);
+        d->tnorain = (uint8_t *)p;
+        FCA_CARVE(p, 2 * sizeof(unsigned long long));
+        dscan = (unsigned long long *)p;
+    }
+#undef FCA_CARVE
+    if (off > bytes) {
+        return fail(-1, "a %d x %d grid needs %zu B of workspace, more than the %zu B reserved",
+                    g->nrows, g->ncols, off, bytes);
+    }
+    d->z = g->z, d->n = g->n, d->C = g->C, d->Rday = g->Rday, d->mask = g->mask;
+    d->h[0] = g->h, d->hmax = g->hmax, d->tmax = g->tmax;
+    /* what fca_hip_run learns from host fields: the LAND roughness range and
+     * the open faces counted into cap_total */
+    if (fca_k_grid_scan(d, dscan, s.st) != 0) {
+        return fail(-2, "grid scan launch failed");
+    }
+    HIP_OR(hipMemcpyAsync(scan, dscan, sizeof(scan), hipMemcpyDeviceToHost, s.st), -2);
+    HIP_OR(hipMemcpyAsync(d->h[1], d->h[0], B, hipMemcpyDeviceToDevice, s.st), -2);
+    HIP_OR(hipStreamSynchronize(s.st), -2);
+    s.n_ok = scan[0] == 0;
+    set_div_flags(d, s.n_ok, 0);
+    s.faces_per_step = count_inner_faces(g->nrows, g->ncols, &cfg->phys) + (long)scan[1];
+    if (sim_start(&s, acc) != 0) {
+        return -2;
+    }
+    if (tm) {
+        tm->t_init += now_s() - t0;
+        tm->t_h2d = tm->t_h2d > 0 ? tm->t_h2d : 0;
+    }
+    t0 = now_s();
+    rc = sim_run_hooked(&s, NULL);
+    if (rc == 0) {
+        rc = fca_hip_sim_sync(&s);
+    }
+    if (tm) {
+        tm->wall_total += now_s() - t0;
+        tm->t_io += s.t_io;
+        tm->t_flux = tm->t_limiter = tm->t_update = tm->t_dt = -1;
+    }
+    if (rc == 0) {
+        t0 = now_s();
+        rc = read_ctrl(&s, &c);
+        if (rc == 0 && c.hbuf == 1) { /* the last step wrote the workspace's buffer */
+            if (hipMemcpyAsync(g->h, d->h[1], B, hipMemcpyDeviceToDevice, s.st) != hipSuccess ||
+                hipStreamSynchronize(s.st) != hipSuccess) {
+                rc = fail(-2, "depth copy-back failed");
+            }
+        }
+        if (rc == 0) {
+            rc = sim_acc(&s, &c, acc);
+        }
+        if (tm) {
+            tm->t_d2h = (tm->t_d2h > 0 ? tm->t_d2h : 0) + (now_s() - t0);
+        }
+    }
     return rc;
 }
 

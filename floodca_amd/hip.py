@@ -77,6 +77,7 @@ class Backend:
             "fca_hip_batch_open_crops": (C.c_void_p, [C.c_void_p, C.c_int,
                                                       C.POINTER(a.FcaHipCrop), C.POINTER(Cf),
                                                       C.POINTER(a.FcaHipOpts)]),
+            "fca_hip_run_noalloc": (C.c_int, [G, Cf, Ac, Tm, C.c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -102,6 +103,27 @@ class Backend:
         g = grid.fca_grid(self.a)
         self.check(self.lib.fca_hip_run(C.byref(g), C.byref(cfg), C.byref(acc), C.byref(tm)),
                    "fca_hip_run")
+        return tm
+
+
+    def run_noalloc(self, fields: dict, nrows: int, ncols: int, dx: float, cfg, acc,
+                    stream=None):
+        """fca_hip_run_noalloc on fields already in device memory: ``fields``
+        maps z, h, n, C, mask, Rday, hmax, tmax to device addresses (ints, e.g.
+        a tensor's data_ptr()). h, hmax and tmax receive the results in place;
+        returns the FcaTimers."""
+        a = self.a
+        g = a.FcaGrid()
+        g.nrows, g.ncols, g.dx = nrows, ncols, dx
+        g.r0, g.r1, g.c0, g.c1 = 0, nrows, 0, ncols
+        P = C.POINTER(a.real)
+        for k in ("z", "h", "n", "C", "Rday", "hmax", "tmax"):
+            setattr(g, k, C.cast(C.c_void_p(fields[k]), P))
+        g.mask = C.cast(C.c_void_p(fields["mask"]), C.POINTER(C.c_uint8))
+        tm = self.a.FcaTimers()
+        tm.t_h2d = tm.t_d2h = tm.t_halo = tm.t_allreduce = tm.bytes_per_step = -1
+        self.check(self.lib.fca_hip_run_noalloc(C.byref(g), C.byref(cfg), C.byref(acc),
+                                                C.byref(tm), stream), "fca_hip_run_noalloc")
         return tm
 
 

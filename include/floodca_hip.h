@@ -358,6 +358,29 @@ typedef struct FcaHipCrop {
 FcaHipBatch *fca_hip_batch_open_crops(const FcaHipBase *base, int n, const FcaHipCrop *crops,
                                       const FcaRunCfg *const *cfg, const FcaHipOpts *opts);
 
+/* ---- Entry point that allocates nothing ---------------------------------- */
+
+/* Largest grid, in cells, that fca_hip_run_noalloc accepts (2048 x 2048). */
+#define FCA_HIP_NOALLOC_MAX_CELLS (1 << 22)
+
+/* No counterpart in the reference. fca_hip_run for callers that cannot
+ * allocate device memory: tight serving loops and harnesses that forbid
+ * hipMalloc. g's z, h, n, C, mask, Rday, hmax and tmax are DEVICE pointers
+ * owned by the caller (h, hmax, tmax are read and receive the results, the
+ * others are only read); the run is fca_hip_run's, with the same kernels,
+ * results bit for bit, accumulators and timers (nothing crosses PCIe: t_h2d
+ * adds nothing, t_d2h times the final read-back of the accumulators).
+ * stream is a hipStream_t (NULL: the default stream); the call returns once
+ * the run is complete. It calls no hipMalloc, hipFree, stream or graph
+ * creation, and no host allocation: its workspace is one array per device,
+ * reserved with the library's code object (about 2 * 4 * 2^22 B + 4 MiB in
+ * float, twice the per-cell part in double) and held until the process ends.
+ * Calls that overlap on one device share that array and race. Rejected with
+ * -1 (message in fca_hip_last_error()): more than FCA_HIP_NOALLOC_MAX_CELLS
+ * cells, a NULL field, K other than 4 or 8, alpha*dx <= 0, multi-day
+ * forcing (its day files are read on the host). -2 on a device error. */
+int fca_hip_run_noalloc(FcaGrid *g, FcaRunCfg *cfg, FcaAcc *acc, FcaTimers *tm, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

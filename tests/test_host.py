@@ -111,3 +111,26 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setenv("FCA_HIP_LIB", str(tmp_path / "absent.so"))
     with pytest.raises(hip.HipError, match="not built"):
         hip.Backend("f32")
+
+
+def test_noalloc_entry_point_rejects_before_touching_the_device():
+    """fca_hip_run_noalloc checks its arguments before any device call: a grid
+    beyond FCA_HIP_NOALLOC_MAX_CELLS, a NULL field and multi-day forcing are
+    refused with a message (the addresses given are never dereferenced)."""
+    fields = ("z", "h", "n", "C", "Rday", "hmax", "tmax", "mask")
+    b = backend("f32")
+    a = b.a
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, 90.0)
+    acc = a.FcaAcc()
+    with pytest.raises(HipError, match="exceeds the 4194304 cells"):
+        b.run_noalloc({k: 1 << 20 for k in fields}, 2049, 2048, 90.0, cfg, acc)
+    with pytest.raises(HipError, match="NULL"):
+        b.run_noalloc({k: 0 for k in fields}, 16, 16, 90.0, cfg, acc)
+    cfg.n_force_days = 2
+    with pytest.raises(HipError, match="forcing"):
+        b.run_noalloc({k: 1 << 20 for k in fields}, 16, 16, 90.0, cfg, acc)
+    cfg.n_force_days = 0
+    cfg.phys.K = 6
+    with pytest.raises(HipError, match="K must be 4 or 8"):
+        b.run_noalloc({k: 1 << 20 for k in fields}, 16, 16, 90.0, cfg, acc)
