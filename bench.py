@@ -243,14 +243,16 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
     dbase = E.device_base(base, prec)
     cells = sum(s.side * s.side for s in scen)
     t_base = time.time() - t0
-    strip, tune = E.autotune_strip_crops(dbase, scen, base.dx, prec)
+    (strip, tpw), tune = E.autotune_tile_crops(dbase, scen, base.dx, prec)
     ph = {}
-    t_b, r_b = E.run_batched_crops(dbase, scen, base.dx, prec, strip_rows=strip, phases=ph)
+    t_b, r_b = E.run_batched_crops(dbase, scen, base.dx, prec, strip_rows=strip, phases=ph,
+                                   tiles_per_wave=tpw)
     rng = np.random.default_rng(seed + 1)
     idx = sorted(rng.choice(n_scen, size=min(n_alone, n_scen), replace=False).tolist())
     sub = [scen[i] for i in idx]
     t_a, r_a = E.run_alone_crops(dbase, sub, base.dx, prec)
-    t_s, r_s = E.run_batched_crops(dbase, sub, base.dx, prec, strip_rows=strip)
+    t_s, r_s = E.run_batched_crops(dbase, sub, base.dx, prec, strip_rows=strip,
+                                   tiles_per_wave=tpw)
     dbase.close()
     same = r_a == r_s == [r_b[i] for i in idx]
     steps = [r[2] for r in r_b]
@@ -258,7 +260,8 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
                                  == A.FCA_LAND)) for sc in scen]
     land_updates = float(sum(n * k for n, k in zip(land, steps)))
     print(f"[bench] ensemble: {n_scen} scenarios ({cells / 1e6:.0f} M cells), base on the "
-          f"device in {t_base:.1f}s, strip {strip} ({ {k: round(v, 3) for k, v in tune.items()} }), "
+          f"device in {t_base:.1f}s, strip {strip} x {tpw} tiles per wavefront "
+          f"({ {k: round(v, 3) for k, v in tune.items()} }), "
           f"batched {t_b:.2f}s ({ {k: round(v, 2) for k, v in ph.items()} }); "
           f"sample of {len(idx)}: alone {t_a:.2f}s, "
           f"batched {t_s:.2f}s, identical={same}", file=sys.stderr, flush=True)
@@ -276,6 +279,8 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
         "land_cell_updates_per_s_batched_run": land_updates / ph["run"],
         "scenarios": n_scen, "sample": len(idx), "cells": cells,
         "steps_min_max": [min(steps), max(steps)], "strip_rows": strip,
+        "tiles_per_wave": tpw,
+        "autotune_s": {k: round(v, 4) for k, v in tune.items()},
         "workload": "crops of the 16384^2 DEM, side U[256, 2048], built on the device; "
                     "triangular storm D U[2, 12] h, 50 mm/h x lognormal(0, 0.5); Manning x "
                     "U[0.8, 1.2] per class; T_sim = D; FP32, K=4",

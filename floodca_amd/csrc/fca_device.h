@@ -39,7 +39,10 @@ typedef struct FcaDevCtrl {
     int want_day;          /* with _PAUSE: the day to load */
     int nrec;              /* step records written */
     int hbuf;              /* which of the two depth buffers holds h */
+    int armed;             /* a step was taken since the state was uploaded */
+    int pad_;
     double dt_base;        /* held step, in the run's precision */
+    double tmax_last;      /* tmax value of the last step taken, (real)(t + dt) */
 } FcaDevCtrl;
 
 enum { FCA_CTL_RUN = 0, FCA_CTL_DONE = 1, FCA_CTL_PAUSE = 2 };
@@ -783,8 +786,10 @@ template <class R>
 struct StepVals {
     double t, F, t_new;
     R dt, dt_base, tmax_val, rain_f, den, rden;
+    R tmax_prev; /* tmax value of the previous step (valid when armed) */
     int active, hit, need_cand, step;
     int den_fast; /* den admits div_step_fast (step_den_fast) */
+    int armed;    /* a step was taken since the state was uploaded */
 };
 
 /* rain.c:13-28 */
@@ -888,6 +893,8 @@ FCA_HD void ctrl_eval(const FcaDevCtrl &in, R cand, const FcaCtlParams &c, const
     sv.need_cand = 0;
     sv.step = in.step;
     sv.t = in.t;
+    sv.armed = in.armed;
+    sv.tmax_prev = (R)in.tmax_last;
     if (in.state != FCA_CTL_RUN) {
         return;
     }
@@ -955,6 +962,8 @@ FCA_HD void ctrl_eval(const FcaDevCtrl &in, R cand, const FcaCtlParams &c, const
     sv.need_cand = ((in.step + 1) % (c.U > 0 ? c.U : 1)) == 0;
     nx.t = sv.t_new;
     nx.step = in.step + 1;
+    nx.armed = 1;
+    nx.tmax_last = (double)sv.tmax_val;
 }
 
 /* Order-preserving key of a time-step candidate for an unsigned atomic min

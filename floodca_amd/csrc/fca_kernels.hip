@@ -112,7 +112,9 @@ struct GpuLanes {
     {
         return __builtin_amdgcn_ballot_w64(s.*f != 0) != 0;
     }
-    /* every lane's dst = the largest src over the wavefront */
+    /* every lane's dst = the largest src over the wavefront; every lane holds
+     * the same value after the butterfly, so readfirstlane is exact and lets
+     * the compiler keep dst in a scalar register */
     __device__ __forceinline__ void wave_max_to(R WalkLane<R>::*dst, R WalkLane<R>::*src)
     {
         R v = s.*src;
@@ -120,7 +122,18 @@ struct GpuLanes {
             const R o = __shfl_xor(v, off, 64);
             v = (o > v) ? o : v;
         }
-        s.*dst = v;
+        s.*dst = uniform(v);
+    }
+    __device__ __forceinline__ static float uniform(float v)
+    {
+        return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+    }
+    __device__ __forceinline__ static double uniform(double v)
+    {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffll));
+        const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
     }
 };
 
@@ -276,6 +289,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
         wa.nr = a.nr, wa.nc = a.nc, wa.strip_rows = a.strip_rows;
         wa.p = a.p;
         wa.sv = sv;
+        wa.lgg = speed_log_half_g<R>(a.p.g);
         const uint8_t *prev = a.tflags + (size_t)((sv.step + 1) & 1) * a.ntiles;
         uint8_t *cur = a.tflags + (size_t)(sv.step & 1) * a.ntiles;
         for (int t = tile; t < a.ntiles; t += W) {
@@ -328,8 +342,14 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
     }
 }
 
+/* Wavefronts per SIMD the walk is compiled for. The K=4 walk without the
+ * quiet-tile bookkeeping fits four (at most 128 VGPRs, no spills), whose
+ * fourth wavefront hides row-load latency; the others need more registers
+ * than that and run three. */
+#define FCA_FUSED_OCCUPANCY(K8, QT) __attribute__((amdgpu_waves_per_eu((K8) || (QT) ? 3 : 4)))
+
 template <int K8, int QT>
-__global__ void __launch_bounds__(FCA_BLOCK) k_fused(FusedArgs a)
+__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_OCCUPANCY(K8, QT) k_fused(FusedArgs a)
 {
     fused_body<K8, QT>(a, blockIdx.x, gridDim.x);
 }
@@ -349,7 +369,7 @@ struct ScenDev {
 };
 
 template <int K8>
-__global__ void __launch_bounds__(FCA_BLOCK)
+__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_OCCUPANCY(K8, 0)
     k_fused_batch(const ScenDev *scen, const int2 *wgs, long long k)
 {
     const int2 e = wgs[blockIdx.x];
@@ -730,6 +750,26 @@ __global__ void __launch_bounds__(FCA_BLOCK)
     }
 }
 
+/* Completes hmax and tmax after the fused walk, which stores them only when a
+ * cell stops rising: a LAND cell whose depth exceeds its stored hmax rose in
+ * the last step taken, so its hmax is its depth and its tmax that step's
+ * t + dt (tlast). Leaves every other cell as it is, so it may run at any
+ * point between steps, any number of times. */
+__global__ void __launch_bounds__(FCA_BLOCK)
+    k_hmax_finalize(const R *h, R *hmax, R *tmax, const uint8_t *mask, size_t N, R tlast)
+{
+    for (size_t i = (size_t)blockIdx.x * FCA_BLOCK + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * FCA_BLOCK) {
+        if (mask[i] == FCA_LAND) {
+            const R v = h[i];
+            if (v > hmax[i]) {
+                hmax[i] = v;
+                tmax[i] = tlast;
+            }
+        }
+    }
+}
+
 /* Fills n 32-bit words with v (used where a caller might capture the work
  * into a graph, which must not hold memset nodes). */
 __global__ void __launch_bounds__(FCA_BLOCK) k_fill_u32(uint32_t *p, size_t n, uint32_t v)
@@ -906,6 +946,10 @@ extern "C" void fca_k_geometry(FcaDev *d)
     d->nstrips = (d->nr + d->strip_rows - 1) / d->strip_rows;
     d->ntiles = d->ngroups * d->nstrips;
     d->nblocks = (d->ntiles + FCA_BLOCK / 64 - 1) / (FCA_BLOCK / 64);
+    if (d->tiles_per_wave > 1) {
+        const int per_block = (FCA_BLOCK / 64) * d->tiles_per_wave;
+        d->nblocks = (d->ntiles + per_block - 1) / per_block;
+    }
     if (d->max_blocks > 0 && d->nblocks > d->max_blocks) {
         d->nblocks = d->max_blocks;
     }
@@ -1125,5 +1169,14 @@ extern "C" int fca_k_grid_scan(const FcaDev *d, unsigned long long *out, hipStre
     }
     hipLaunchKernelGGL(k_grid_scan, dim3(g < 4096 ? g : 4096), dim3(FCA_BLOCK), 0, st, d->n,
                        d->mask, d->nr, d->nc, d->phys.boundary == FCA_BND_OPEN, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int fca_k_hmax_finalize(const FcaDev *d, int hbuf, double tlast, hipStream_t st)
+{
+    const size_t N = (size_t)d->nr * d->nc;
+    unsigned g = blocks_for(N);
+    hipLaunchKernelGGL(k_hmax_finalize, dim3(g < 65536 ? g : 65536), dim3(FCA_BLOCK), 0, st,
+                       d->h[hbuf], d->hmax, d->tmax, d->mask, N, (R)tlast);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -39,8 +39,9 @@ typedef struct FcaDev {
     /* the fused walk may take its LAND form (fca_land_form_ok and the
      * division flags it relies on) */
     int land_form;
-    /* fused geometry; max_blocks caps the persistent grid (0 = one wave per tile) */
-    int strip_rows, ngroups, nstrips, ntiles, nblocks, max_blocks;
+    /* fused geometry; max_blocks caps the persistent grid (0 = one wave per tile),
+     * tiles_per_wave > 1 sizes it to walk that many tiles per wavefront */
+    int strip_rows, ngroups, nstrips, ntiles, nblocks, max_blocks, tiles_per_wave;
     /* quiet-tile skipping: flags [2][ntiles], rain-free tiles [ntiles], on/off */
     uint8_t *tflags, *tnorain;
     int skip;
@@ -62,6 +63,10 @@ typedef struct FcaDev {
 /* Device address and size of this device's workspace (hipGetSymbolAddress
  * once per device). Returns 0, or -1 if the address cannot be had. */
 int fca_k_noalloc_scratch(void **ptr, size_t *bytes);
+/* hmax = h and tmax = tlast on the LAND cells whose depth h[hbuf] exceeds
+ * their stored hmax (the fused walk stores hmax and tmax only when a cell
+ * stops rising); the controller record's tmax_last is tlast. */
+int fca_k_hmax_finalize(const FcaDev *d, int hbuf, double tlast, hipStream_t st);
 /* Zeroes bytes (rounded up to a multiple of 4) at p with a fill kernel. */
 int fca_k_zero(void *p, size_t bytes, hipStream_t st);
 /* out[0] = nonzero if a LAND roughness leaves [1e-12, 2^100]; out[1] = open

@@ -11,6 +11,7 @@
  *      limiter s (model.c:158-238, 488-502);
  *   C. updates row r-2: upwind-scaled net inflow, rain, clamp, hmax/tmax and
  *      the FP64 balance terms (model.c:241-319, 511-597);
+ *      hmax and tmax are stored only when a cell stops rising (see below);
  *   D. bounds the next step's stable time step on row r-3 from the new
  *      depths (model.c:111-156, physics.h:158-217).
  * Rows r-1..r-4 stay in a register ring, so every field crosses HBM once per
@@ -31,6 +32,19 @@
  * work-item and pull_left/pull_right are DPP wavefront shifts; the CPU
  * emulation runs 64 lanes in lockstep with array shifts, so the CPU test
  * suite checks the same dataflow bit for bit against the oracle.
+ *
+ * hmax and tmax. The reference sets hmax = h and tmax = t + dt whenever a
+ * step leaves h > hmax (model.c:585-590), which under steady rain is every
+ * LAND cell on every step. The walk stores them only when a cell stops
+ * rising, keeping the invariant that after any step the true hmax is
+ * max(stored hmax, h) and that h > stored hmax exactly when the cell rose
+ * in that step (its true tmax then being that step's t + dt). So a cell with
+ * h_old > stored hmax rose in the previous step; if the new depth does not
+ * exceed h_old it has stopped, and h_old and the previous step's tmax value
+ * are stored. The first step after an upload makes no such store (the
+ * caller's h may exceed its hmax before any step); the host finalises the
+ * cells still rising (hmax = h, tmax = the last step's value) before any
+ * read of hmax or tmax.
  *
  * Lane validity: faces are valid on lanes 0..62, limiters on 1..62, new
  * depths on 2..61 and time-step candidates on 3..60, which are the lanes a
@@ -57,6 +71,7 @@ struct WalkArgs {
     int strip_rows;
     PhysK<R> p;
     StepVals<R> sv;
+    R lgg; /* (1/2) lg g for speed_may_exceed_fast (speed_log_half_g) */
 };
 
 /* Per-lane state of the walk (registers on the GPU). Suffix k = row r-k. */
@@ -85,8 +100,9 @@ struct WalkLane {
     int pm1, pm2;
     R pz, ph, pn, pcr, phm, pcr1, phm1;
     /* stores of row r-3 deferred to the next row's memory block: bit 0 h,
-     * bit 1 hmax and tmax */
+     * bit 1 hmax (= hst, the cell's depth before the step) and tmax */
     int stq;
+    R hst;
     /* this tile's output cells: some depth >= tau_h after the step (qwet),
      * some h, hmax or tmax changed by the step (qchg) */
     int qwet, qchg;
@@ -94,7 +110,8 @@ struct WalkLane {
     double v_rain, clamp, v_sea, v_bnd;
     long long cap;
     R dtc, vmax, vthr;
-    R lthr, lgg; /* log2 forms of vthr and sqrt(g) for speed_may_exceed_fast */
+    R lthr; /* log2 form of vthr for speed_may_exceed_fast (vthr and lthr are
+             * the same on every lane of a wavefront) */
     /* stage D: largest |level difference| per face length, and whether the
      * exact speed is needed */
     R dax, dad;
@@ -201,8 +218,11 @@ FCA_HDI int speed_may_exceed(R h, R n, R s_hi, int flat, R thr, R g)
 FCA_HDI float speed_log_thr(float thr)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return (thr >= 1e-6f && thr <= 0x1p100f) ? __builtin_amdgcn_logf(thr) - 0x1p-10f
-                                              : -INFINITY;
+    /* thr is the same on every lane: so is the result, which readfirstlane
+     * lets the compiler keep in a scalar register */
+    const float l = (thr >= 1e-6f && thr <= 0x1p100f) ? __builtin_amdgcn_logf(thr) - 0x1p-10f
+                                                       : -INFINITY;
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(l)));
 #else
     (void)thr;
     return -INFINITY;
@@ -241,8 +261,9 @@ FCA_HDI R speed_log_half_g(R g)
 #if defined(__HIP_DEVICE_COMPILE__)
     if (sizeof(R) == 4) {
         /* +inf (no skip by the critical speed) unless g is a normal float */
-        return (g >= (R)0x1p-126 && g <= (R)0x1p127) ? (R)(0.5f * __builtin_amdgcn_logf((float)g))
-                                                    : (R)INFINITY;
+        const float l = (g >= (R)0x1p-126 && g <= (R)0x1p127)
+                            ? 0.5f * __builtin_amdgcn_logf((float)g) : INFINITY;
+        return (R)__int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(l)));
     }
 #endif
     (void)g;
@@ -549,6 +570,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
             if (!LANDF && outC && s.outcol) {
                 s.v_sea += sea;
             }
+            s.hst = s.h2; /* the depth a stop-rising store would record */
             if (!land) {
                 s.hn2 = s.h2;
                 return;
@@ -564,12 +586,18 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 }
                 s.hn2 = hnew;
                 if (outC && s.outcol) {
+                    /* rose in the previous step (h_old > stored hmax) and
+                     * not in this one: store hmax = h_old and its tmax */
+                    const int stop = sv.armed && s.h2 > s.phm && !(hnew > s.h2);
                     s.v_rain += (double)A * (double)rain;
                     s.clamp += clamp_here;
-                    s.stq = (hnew > s.phm) ? 3 : 1;
+                    s.stq = stop ? 3 : 1;
                     if (QT) {
+                        /* changed: h, a store, or the true hmax (h_new >
+                         * stored hmax also where h does not move, as on the
+                         * first step from a caller's h above its hmax) */
                         s.qwet |= !(hnew < p.tau_h);
-                        s.qchg |= !same_bits(hnew, s.h2) || hnew > s.phm;
+                        s.qchg |= !same_bits(hnew, s.h2) || stop || hnew > s.phm;
                     }
                 }
             }
@@ -657,7 +685,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
             }
             s_hi = s_hi * ((R)1 + (R)0x1p-20);
             s.dneed = speed_may_exceed_fast<R>(s.hn3, s.n3, s_hi, ax == (R)0 && ad == (R)0,
-                                               s.vthr, s.lthr, s.lgg, p.g);
+                                               s.vthr, s.lthr, a.lgg, p.g);
         });
         if (lp.any(&L::dneed)) {
             lp.each([&](L &s, int) {
@@ -706,7 +734,6 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         s.colok = s.c >= 0 && s.c < nc;
         s.outcol = s.colok && lane >= FCA_HALO && lane < FCA_WAVE - FCA_HALO;
         s.boff = (uint32_t)s.c * (uint32_t)sizeof(R);
-        s.lgg = speed_log_half_g<R>(a.p.g);
         s.z0 = s.z1 = s.z2 = s.z3 = s.z4 = (R)0;
         s.m0 = s.m1 = s.m2 = s.m3 = s.m4 = FCA_NONE;
         s.h0 = s.h1 = s.h2 = (R)0;
@@ -718,6 +745,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         s.s1 = s.s2 = s.s3 = (R)1;
         s.pcr = s.phm = s.pcr1 = s.phm1 = (R)0;
         s.stq = 0;
+        s.hst = (R)0;
         s.qwet = s.qchg = 0;
         s.pz = s.ph = s.pn = (R)0;
         s.pm1 = s.pm2 = FCA_NONE;
@@ -758,7 +786,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             const int qs = r - 3, oks = qs >= 0 && qs < nr;
             const size_t ros = (size_t)(oks ? qs : 0) * (size_t)nc;
             R *hos = a.h_out + ros, *hms = a.hmax + ros, *tms = a.tmax + ros;
-            const R tval = a.sv.tmax_val;
+            const R tval = a.sv.tmax_prev;
             lp.each([&](L &s, int) {
                 s.m0 = s.pm1;
                 if (s.m0 == FCA_LAND) {
@@ -773,7 +801,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
                 if (s.stq) {
                     st_row(hos, s.boff, s.hn3);
                     if (s.stq & 2) {
-                        st_row(hms, s.boff, s.hn3);
+                        st_row(hms, s.boff, s.hst);
                         st_row(tms, s.boff, tval);
                     }
                     s.stq = 0;
@@ -844,8 +872,8 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             if (s.stq) {
                 st_row(hos, s.boff, s.hn3);
                 if (s.stq & 2) {
-                    st_row(hms, s.boff, s.hn3);
-                    st_row(tms, s.boff, a.sv.tmax_val);
+                    st_row(hms, s.boff, s.hst);
+                    st_row(tms, s.boff, a.sv.tmax_prev);
                 }
                 s.stq = 0;
             }

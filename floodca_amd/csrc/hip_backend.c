@@ -298,6 +298,7 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
     s->no_land_form = o && (o->flags & FCA_HIP_NO_LAND_FORM);
     set_div_flags(d, 0, s->no_land_form);
     d->max_blocks = (o && o->max_blocks > 0) ? o->max_blocks : 0;
+    d->tiles_per_wave = (o && o->tiles_per_wave > 1) ? o->tiles_per_wave : 1;
     fca_k_geometry(d);
     s->N = N;
     s->batch = (o && o->graph_batch > 0) ? o->graph_batch : 30;
@@ -745,6 +746,18 @@ static int write_ctrl(FcaHipSim *s, const FcaDevCtrl *c)
     return 0;
 }
 
+/* Completes hmax and tmax on the device for the state the record c
+ * describes (the fused walk stores them only when a cell stops rising);
+ * nothing to do before the first step of the session. Enqueued on the
+ * session's stream. */
+static int sim_finalize(FcaHipSim *s, const FcaDevCtrl *c)
+{
+    if (c->armed && fca_k_hmax_finalize(&s->d, c->hbuf, c->tmax_last, s->st) != 0) {
+        return fail(-2, "hmax finalise launch failed");
+    }
+    return 0;
+}
+
 /* Loads forcing day `day` (float32 raster) into Rday, scaled by lambda
  * (model.c:338-358, grid.c:239-254). Returns 0, or -1 if unreadable. */
 static int reload_day(FcaHipSim *s, int day)
@@ -948,9 +961,10 @@ int fca_hip_sim_read(FcaHipSim *s, real *h, real *hmax, real *tmax, FcaAcc *acc)
     if (!s) {
         return fail(-1, "NULL session");
     }
-    if (read_ctrl(s, &c) != 0) {
+    if (read_ctrl(s, &c) != 0 || sim_finalize(s, &c) != 0) {
         return -2;
     }
+    HIP_OR(hipStreamSynchronize(s->st), -2);
     if (h) {
         HIP_OR(hipMemcpy(h, s->d.h[c.hbuf], B, hipMemcpyDeviceToHost), -2);
     }
@@ -1155,6 +1169,9 @@ int fca_hip_run_noalloc(FcaGrid *g, FcaRunCfg *cfg, FcaAcc *acc, FcaTimers *tm, 
     if (rc == 0) {
         t0 = now_s();
         rc = read_ctrl(&s, &c);
+        if (rc == 0) {
+            rc = sim_finalize(&s, &c);
+        }
         if (rc == 0 && c.hbuf == 1) { /* the last step wrote the workspace's buffer */
             if (hipMemcpyAsync(g->h, d->h[1], B, hipMemcpyDeviceToDevice, s.st) != hipSuccess ||
                 hipStreamSynchronize(s.st) != hipSuccess) {
@@ -1317,7 +1334,8 @@ int fca_hip_ens_run(FcaHipEns *e, FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc,
         tm->wall_total += now_s() - t0;
         tm->t_flux = tm->t_limiter = tm->t_update = tm->t_dt = -1;
     }
-    if (rc != 0 || read_ctrl(s, &c) != 0 || sim_acc(s, &c, acc) != 0) {
+    if (rc != 0 || read_ctrl(s, &c) != 0 || sim_finalize(s, &c) != 0 ||
+        sim_acc(s, &c, acc) != 0) {
         return -2;
     }
     t0 = now_s();
@@ -1762,9 +1780,15 @@ int fca_hip_batch_run(FcaHipBatch *b)
             return -2;
         }
     }
+    /* hmax and tmax complete on the device before anything reads them (the
+     * flooded counts, count rasters and the device fields handed out) */
     for (i = 0; i < b->n; i++) {
         b->sims[i]->k = b->k_done[i];
+        if (sim_finalize(b->sims[i], &b->h_ctrl[2 * i + (int)(b->k_done[i] & 1)]) != 0) {
+            return -2;
+        }
     }
+    HIP_OR(hipStreamSynchronize(b->st), -2);
     b->wall += now_s() - t0;
     return 0;
 }

@@ -29,7 +29,10 @@
  * ensemble session fca_hip_ens_*, the calls the reference's rank makes;
  * --batch B runs them B at a time in one batched session (fca_hip_batch_*),
  * each scenario with its own time step, for throughput. Both give the same
- * results bit for bit (the batch's per-scenario arithmetic is a session's).
+ * depths, counts and step counts bit for bit (the batch's per-scenario
+ * arithmetic is a session's); the FP64 balance sums, added per workgroup,
+ * may differ in their last digits, as the batch walks strips of its own
+ * height.
  *
  * Compiled against the reference's src/common (oracle/build_hip_main.sh), so
  * grid loading, rain files, accumulators and the bench CSV row are the

@@ -252,7 +252,10 @@ typedef struct FcaHipOpts {
                         * (default 0: one strip per wavefront, hardware-dispatched) */
     int flags;         /* FCA_HIP_NO_LAND_FORM: the fused walk never takes its LAND
                         * form (the same results; for comparisons) */
-    int reserved[2];
+    int tiles_per_wave; /* fused: tiles each wavefront walks per launch (default 1):
+                         * k > 1 launches a k-th of the workgroups, each wavefront
+                         * walking k strips in turn (more cells per lane) */
+    int reserved;
 } FcaHipOpts;
 
 /* flags: FCA_HIP_NO_LAND_FORM as above; FCA_HIP_NO_SKIP walks every tile

@@ -280,6 +280,7 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
             wa.z = g->z, wa.n = g->n, wa.cr = cr, wa.mask = g->mask;
             wa.h_in = h[hb], wa.h_out = h[1 - hb], wa.hmax = g->hmax, wa.tmax = g->tmax;
             wa.nr = nr, wa.nc = nc, wa.strip_rows = strip_rows, wa.p = p, wa.sv = sv;
+            wa.lgg = speed_log_half_g<R>(p.g);
             for (int t = 0; t < ngroups * nstrips; t++) {
                 static CpuLanes lp;
                 if (t == 0 && c.step == acc0.step) {
@@ -330,6 +331,17 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
         c = nx;
     }
     memcpy(g->h, h[hb], N * sizeof(R));
+    /* the walk stores hmax and tmax only when a cell stops rising: the cells
+     * still rising take their depth and the last step's tmax (as the host
+     * library's finalise step) */
+    if (c.armed) {
+        for (i = 0; i < N; i++) {
+            if (g->mask[i] == FCA_LAND && g->h[i] > g->hmax[i]) {
+                g->hmax[i] = g->h[i];
+                g->tmax[i] = (R)c.tmax_last;
+            }
+        }
+    }
     acc->cap_total = acc0.cap_total + faces * (long)(c.step - acc0.step);
     acc->dt_min_hits = acc0.dt_min_hits + (long)c.dt_min_hits;
     acc->t = c.t;

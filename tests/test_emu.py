@@ -139,3 +139,46 @@ def test_forcing_days_match_oracle(prec, days_present, tmp_path):
     assert [r.dt for r in log_o] == [r.dt for r in log_e]
     assert acc_o.step == acc_e.step and acc_o.t == acc_e.t == 86400.0 * 3
     assert acc_o.force_day == acc_e.force_day == days_present - 1
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("hyeto", [A.FCA_HYETO_UNIFORM, A.FCA_HYETO_TRIANGLE])
+def test_hmax_below_the_initial_depth(prec, hyeto):
+    """The walk stores hmax and tmax only when a cell stops rising, and the
+    first step after an upload stores none (a caller's h may exceed its
+    hmax). Warm starts whose hmax lies below h on a third of the cells --
+    wet and dry, on tiles that go quiet and tiles that stay busy -- and whose
+    tmax is nonzero end with the oracle's hmax and tmax bit for bit, with and
+    without quiet-tile skipping."""
+    a, g, cfg = make_case(90, 250, prec=prec, hyeto=hyeto, T_sim=3000.0, seed=5, h0=0.0,
+                          D_h=0.5, phi=0.3, rday=0.0005)
+    rng = np.random.default_rng(9)
+    land = g.mask == A.FCA_LAND
+    cols = np.arange(g.shape[1])[None, :]
+    wet = land & (rng.random(g.shape) < 0.3) & (cols >= 232)      # the fifth column group
+    g.h[wet] = rng.uniform(0.0, 0.2, wet.sum()).astype(g.h.dtype)
+    g.hmax[...] = g.h
+    low = land & (rng.random(g.shape) < 0.33)
+    g.hmax[low] = (g.h[low] * rng.uniform(0.0, 0.9, low.sum())).astype(g.h.dtype)
+    g.tmax[land] = rng.uniform(0.0, 100.0, land.sum()).astype(g.h.dtype)
+    # no rain on the second column group (columns 58-115, its neighbours dry),
+    # and there depths below tau_h above their hmax: the reference sets
+    # hmax = h and tmax on the first step although h does not move, and the
+    # tile must not go quiet before the walk stores them
+    zone = (cols >= 58) & (cols < 116) & np.ones(g.shape, dtype=bool)
+    g.C[zone] = 0
+    damp = zone & land & (rng.random(g.shape) < 0.2)
+    assert damp.sum() > 100
+    g.h[damp] = a.dtype.type(3e-4)
+    g.hmax[damp] = a.dtype.type(1e-4)
+    o = Oracle(prec)
+    go = g.copy()
+    acc_o, _ = o.run(go, cfg)
+    for skip in (True, False):
+        ge = g.copy()
+        skipped = []
+        emu_run(ge, cfg, o.acc_init(ge, cfg), prec, strip_rows=8, skip=skip, skipped=skipped)
+        for k in ("h", "hmax", "tmax"):
+            assert np.array_equal(getattr(go, k), getattr(ge, k)), (k, skip)
+        if skip and hyeto == A.FCA_HYETO_TRIANGLE:
+            assert skipped[0] > 0  # a 0.5 mm storm over 30 min: dry tiles go quiet

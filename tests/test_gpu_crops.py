@@ -183,3 +183,33 @@ def test_config3_scenarios_match_the_oracle(prec):
             md_ref = max(0.0, float(go.hmax[land].max())) if land.any() else 0.0
             assert md == pytest.approx(md_ref, rel=1e-12 if prec == "f32" else 1e-9)
         b.close()
+
+
+def test_tiles_per_wavefront_leave_results_unchanged():
+    """The autotuner's second dimension: wavefronts walking 2 or 3 tiles per
+    launch (a half or a third of the workgroups) end every scenario with the
+    same steps, flooded counts and largest depth as one tile per wavefront,
+    and with the same depths bit for bit."""
+    base = _base("f32", 512)
+    scens = E.draw(6, 13, base.shape, side=(64, 300), dur=(0.5, 1.0))
+    print("case tiles per wavefront", flush=True)
+    with E.device_base(base) as db:
+        _, one = E.run_batched_crops(db, scens, base.dx, strip_rows=32)
+        for k in (2, 3):
+            _, many = E.run_batched_crops(db, scens, base.dx, strip_rows=32, tiles_per_wave=k)
+            assert many == one, k
+        crops = [E.crop_of(s) for s in scens]
+        cfgs = [E.config_of(s, base.dx) for s in scens]
+        b1 = Batch.from_crops(db, crops, cfgs, strip_rows=32)
+        b3 = Batch.from_crops(db, crops, cfgs, strip_rows=32, tiles_per_wave=3)
+        b1.run()
+        b3.run()
+        for i, s in enumerate(scens):
+            g1 = synth.make_grid(s.side, s.side, seed=1)
+            g3 = g1.copy()
+            b1.read(i, g1)
+            b3.read(i, g3)
+            for k in ("h", "hmax", "tmax"):
+                assert np.array_equal(getattr(g1, k), getattr(g3, k)), (i, k)
+        b1.close()
+        b3.close()

@@ -132,6 +132,9 @@ def test_ensemble_driver_matches_the_oracle(prec, tmp_path):
             assert (hdr["nrows"], hdr["ncols"]) == cnt.shape
             assert hdr["threshold_m"] == THR[j] and hdr["name"] == f"count_{tag}"
         outs[mode] = [ln.split(",") for ln in lines[1:]]
-    # the two modes agree on every field but the wall
+    # the two modes agree on every field but the wall and the last digits of
+    # the mass-balance residual (the FP64 balance terms are summed per
+    # workgroup, and the batch walks strips of another height)
     for a_, b_ in zip(outs["session"], outs["batch"]):
-        assert a_[:11] + a_[12:] == b_[:11] + b_[12:]
+        assert a_[:11] + a_[12:14] + a_[15:] == b_[:11] + b_[12:14] + b_[15:]
+        assert abs(float(a_[14]) - float(b_[14])) < 1e-12
