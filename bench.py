@@ -390,6 +390,64 @@ def sparse_bench(prec: str, size: int = 8192, hours: float = 8.0):
                         f"peaking at 2 h over {hours:g} h, no rain, FP32, K=4"}
 
 
+def sparse_sweep_bench(prec: str, size: int = 8192, fracs=(0.001, 0.01, 0.05, 0.2),
+                       steps: int = 200, depth: float = 0.05):
+    """Quiet-tile skipping against the wet fraction: the sparse-start grid (no
+    rain but the inflow cell's hydrograph) from warm starts in which a disc of
+    LAND around the inflow cell, holding `frac` of all LAND cells, is wet at
+    `depth`; `steps` steps timed with skipping and with every tile walked
+    (results must be identical). The hydrograph itself (7.2 Mm^3) cannot wet
+    more than ~1.9% of this grid's LAND to 1 mm, so larger fractions need a
+    warm start."""
+    a = A.abi(prec)
+    g0 = synth.make_grid(size, size, seed=20261205, dtype=a.dtype)
+    r, c = synth.point_inflow(g0, peak_m3s=500.0, dur_h=8.0)
+    land = g0.mask == A.FCA_LAND
+    rr, cc = np.nonzero(land)
+    d2 = (rr.astype(np.int64) - r) ** 2 + (cc.astype(np.int64) - c) ** 2
+    order = np.argsort(d2, kind="stable")
+    cfg = A.default_cfg(a)
+    A.set_dx(a, cfg, g0.dx)
+    cfg.hyeto, cfg.D_h, cfg.phi = A.FCA_HYETO_TRIANGLE, 8.0, 0.25
+    cfg.T_sim = 1e9
+    cfg.steps = steps
+    res = []
+    for f in fracs:
+        k = int(round(f * rr.size))
+        g = g0.copy()
+        g.h[rr[order[:k]], cc[order[:k]]] = a.dtype.type(depth)
+        g.hmax[...] = g.h
+        out = {}
+        for name, flags in (("skip", 0), ("dense", A.FCA_HIP_NO_SKIP)):
+            gg = g.copy()
+            acc = a.FcaAcc()
+            acc.force_day = -1
+            timer = HipTimer()
+            with Sim(gg, cfg, acc, prec=prec, flags=flags) as sim:
+                stream = C.c_void_p(sim.stream)
+                timer.device_sync()
+                timer.start(stream)
+                sim.run()
+                ms = timer.stop(stream)
+                st = sim.read().step
+            out[name] = (ms, st, gg)
+        same = all(np.array_equal(getattr(out["skip"][2], q), getattr(out["dense"][2], q))
+                   for q in ("h", "hmax", "tmax"))
+        wet_end = float(np.mean(out["skip"][2].h[land] >= 1e-3))
+        res.append({"wet_fraction_start": k / rr.size, "wet_fraction_end": wet_end,
+                    "steps": out["skip"][1], "ms_skip": out["skip"][0],
+                    "ms_dense": out["dense"][0], "speedup": out["dense"][0] / out["skip"][0],
+                    "results_identical": bool(same)})
+        print(f"[bench] sparse sweep {size}^2, {100 * k / rr.size:.2f}% of LAND wet: "
+              f"{out['skip'][1]} steps, skip {out['skip'][0]:.1f} ms, dense "
+              f"{out['dense'][0]:.1f} ms ({out['dense'][0] / out['skip'][0]:.2f}x), "
+              f"identical={same}", file=sys.stderr, flush=True)
+    return {"grid": [size, size], "runs": res,
+            "workload": f"{size}^2, no rain but the inflow cell's hydrograph; warm start with a "
+                        f"disc of LAND around the inflow cell {depth:g} m deep; {steps} steps "
+                        f"with quiet-tile skipping and dense, FP32, K=4"}
+
+
 def grid4096_bench(steps: int = 500, warmup: int = 300):
     """The plain method on the 4096^2 grid of the same generator in FP32 and
     in FP64 (the double-precision mode of the reference's comparison), timed
@@ -627,6 +685,7 @@ def main() -> int:
         line["grid_4096"] = grid4096_bench()
     if args.sparse_size > 0 and prec == "f32" and args.neigh == 4:
         line["sparse_start"] = sparse_bench(prec, args.sparse_size)
+        line["sparse_sweep"] = sparse_sweep_bench(prec, args.sparse_size)
     if args.ens_scenarios > 0 and prec == "f32" and args.neigh == 4:
         line["ensemble"] = ensemble_bench(g, prec, args.ens_scenarios, args.ens_alone)
     if args.day_scenarios > 0 and prec == "f32" and args.neigh == 4:
