@@ -348,10 +348,19 @@ FCA_HD float cbrt_sq_big(float hf)
 FCA_HD double cbrt_sq_big(double hf) { return cbrt_sq(hf); }
 
 #if defined(__HIP_DEVICE_COMPILE__)
+/* Lane mask of a condition. Given a single compare it is that compare's own
+ * result (no vector instruction); a condition built from several compares
+ * costs a select and a compare to turn into a mask, so callers ballot each
+ * compare and combine the masks with scalar & and |. */
+__device__ inline uint64_t lane_mask(bool c)
+{
+    return __builtin_amdgcn_ballot_w64(c);
+}
+
 /* Whether the low 29 bits of a double estimate y (those below float
  * precision) lie within FCA_CBRT_GUARD double ulps of the float rounding
  * midpoint, i.e. (float)y might not be the correctly rounded value. */
-__device__ inline int near_float_midpoint(double y)
+__device__ inline bool near_float_midpoint(double y)
 {
     uint64_t bits;
     __builtin_memcpy(&bits, &y, 8);
@@ -362,10 +371,10 @@ __device__ inline int near_float_midpoint(double y)
 /* cbrt_sq_big without branches: the correctly rounded hf^(2/3) for
  * 2^-15 <= hf <= 2^15 unless *fix is set (near a midpoint, or hf out of that
  * range), where the caller must take cbrt_sq_big. */
-__device__ inline float cbrt_sq_cert(float hf, int &fix)
+__device__ inline float cbrt_sq_cert(float hf, bool &fix)
 {
     const double y1 = cbrt_sq_newton(hf, (double)hf * (double)hf);
-    fix |= near_float_midpoint(y1) | !(hf <= FCA_CBRT_HI);
+    fix = fix || near_float_midpoint(y1) || !(hf <= FCA_CBRT_HI);
     return (float)y1;
 }
 
@@ -378,14 +387,14 @@ __device__ inline float cbrt_sq_cert(float hf, int &fix)
  * x within [2^-100, 2^100] keeps every step normal (the LAND form has
  * h >= 2^-15 and g in [2^-80, 2^80]). Checked against the double expression
  * for every h in [2^-15, 2^15] (tests/test_gpu_math.py). */
-__device__ inline float crit_speed_cert(float g, float h, int &fix)
+__device__ inline float crit_speed_cert(float g, float h, bool &fix)
 {
     const double x = (double)g * (double)h;
     const float s = __builtin_amdgcn_sqrtf((float)x);
     const double sd = (double)s;
     const double r = __builtin_fma(-sd, sd, x);
     const double y = __builtin_fma(r, (double)__builtin_amdgcn_rcpf(2.0f * s), sd);
-    fix |= near_float_midpoint(y);
+    fix = fix || near_float_midpoint(y);
     return (float)y;
 }
 #endif
@@ -597,7 +606,7 @@ FCA_HDI R face_land_exact(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R
 }
 
 /* face_land as the LAND form evaluates it. On the device (float) every lane
- * takes the same straight-line path -- dead faces computed on stand-in
+ * takes the same straight-line path -- dead faces computed on their own
  * inputs and masked at the end, the cube root and the critical speed in
  * their certified branch-free forms (cbrt_sq_cert, crit_speed_cert), Qcap
  * by Markstein's refinement -- and each lane raises a flag where a
@@ -614,42 +623,54 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
         const float dL = la - lb;
         const float adL = __builtin_fabsf(dL);
         const float hf = ((la > lb) ? la : lb) - ((za > zb) ? za : zb);
-        const int live = !((adL < (float)p.tau_l) | (hf < (float)p.tau_h));
+        /* flags are bools combined with && and ||, so each stays a lane mask
+         * and its ballot costs no vector instruction */
+        const bool live = !(adL < (float)p.tau_l) && !(hf < (float)p.tau_h);
+        const uint64_t mlive = lane_mask(!(adL < (float)p.tau_l)) & lane_mask(!(hf < (float)p.tau_h));
         cap = 0;
-        if (__builtin_amdgcn_ballot_w64(live) == 0) {
+        if (mlive == 0) {
             return (R)0;
         }
         {
-            /* dead lanes compute on 1 m stand-ins, inside every fast range */
-            const float hx = live ? hf : 1.0f, ax = live ? adL : 1.0f;
-            int fix = !(ax <= 0x1p80f) | !den_fast;
+            /* dead lanes compute on their own (possibly out-of-range) inputs:
+             * every value they produce, their flags included, is masked by
+             * the live lanes below, and floating-point exceptions do not trap.
+             * mfix collects, one compare at a time, the lanes whose certificate
+             * fails or whose input leaves the ranges the fast forms assume */
+            uint64_t mfix = lane_mask(!(adL <= 0x1p80f)) | (den_fast ? 0ull : ~0ull);
             const float nf = 0.5f * ((float)na + (float)nb);
-            const float q0 = ax * (float)rd;
-            const float sl = __builtin_fmaf(__builtin_fmaf(-q0, (float)d, ax), (float)rd, q0);
-            float v = rcp_in_range(nf) * cbrt_sq_cert(hx, fix) * sqrt_in_range(sl);
+            const float q0 = adL * (float)rd;
+            const float sl = __builtin_fmaf(__builtin_fmaf(-q0, (float)d, adL), (float)rd, q0);
+            const double y1 = cbrt_sq_newton(hf, (double)hf * (double)hf);
+            mfix |= lane_mask(near_float_midpoint(y1)) | lane_mask(!(hf <= FCA_CBRT_HI));
+            float v = rcp_in_range(nf) * (float)y1 * sqrt_in_range(sl);
             float qmag, x, c0, qcap;
             {
                 /* the critical speed only where some lane's v is not clearly
                  * below it (crit_cap's test): a uniform branch */
-                const float gh = (float)p.g * hx;
-                if (__builtin_amdgcn_ballot_w64(live & !(v * v < gh * (1.0f - 0x1p-16f))) != 0) {
-                    const float vcr = crit_speed_cert((float)p.g, hx, fix);
+                const float gh = (float)p.g * hf;
+                if ((lane_mask(!(v * v < gh * (1.0f - 0x1p-16f))) & mlive) != 0) {
+                    bool cfix = false;
+                    const float vcr = crit_speed_cert((float)p.g, hf, cfix);
+                    mfix |= lane_mask(cfix);
                     if (v > vcr) {
                         v = vcr;
                     }
                 }
             }
-            qmag = v * hx * (float)p.w;
-            x = ax * (float)p.A;
-            fix |= !(x >= 0x1p-80f && x <= 0x1p80f);
+            qmag = v * hf * (float)p.w;
+            x = adL * (float)p.A;
+            mfix |= lane_mask(!(x >= 0x1p-80f)) | lane_mask(!(x <= 0x1p80f));
             c0 = x * (float)rden;
             qcap = __builtin_fmaf(__builtin_fmaf(-c0, (float)den, x), (float)rden, c0);
-            cap = live & (qcap < qmag);
+            cap = (live && qcap < qmag) ? 1 : 0;
             qmag = (qcap < qmag) ? qcap : qmag;
             qmag = (dL > 0.0f) ? qmag : -qmag;
-            if (__builtin_amdgcn_ballot_w64(live & fix) != 0) {
+            if ((mfix & mlive) != 0) {
                 FCA_COLD_PATH();
-                if (live & fix) {
+                const uint64_t lane_bit = 1ull << __builtin_amdgcn_mbcnt_hi(
+                                              ~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+                if ((mfix & mlive & lane_bit) != 0) {
                     int cx;
                     const R q = face_land_exact<R>(za, la, na, zb, lb, nb, d, rd, dt, den, rden,
                                                    den_fast, p, cx);
@@ -695,9 +716,9 @@ FCA_HDI R limiter_s_wave(R out_vol, R h, R A)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     const R hold = h * A;
-    const int lim = out_vol > hold && out_vol > (R)0;
+    const bool lim = out_vol > hold && out_vol > (R)0;
     R s = (R)1;
-    if (__builtin_amdgcn_ballot_w64(lim) != 0) {
+    if ((lane_mask(out_vol > hold) & lane_mask(out_vol > (R)0)) != 0) {
         const R q = hold / out_vol;
         s = lim ? q : s;
     }
@@ -717,9 +738,10 @@ FCA_HDI float div_c_wave(float a, float d, float rd)
     const float aa = __builtin_fabsf(a);
     const float q0 = a * rd;
     float q = __builtin_fmaf(__builtin_fmaf(-q0, d, a), rd, q0);
-    const int slow = !((aa >= 0x1p-80f && aa <= 0x1p80f) || aa == 0.0f);
+    const bool slow = !((aa >= 0x1p-80f && aa <= 0x1p80f) || aa == 0.0f);
     q = (aa == 0.0f) ? q0 : q;
-    if (__builtin_amdgcn_ballot_w64(slow) != 0) {
+    if (((lane_mask(!(aa >= 0x1p-80f)) | lane_mask(!(aa <= 0x1p80f))) & ~lane_mask(aa == 0.0f)) !=
+        0) {
         const float qd = a / d;
         q = slow ? qd : q;
     }

@@ -97,15 +97,16 @@ struct GpuLanes {
         __asm__ volatile("" ::"v"(s.lv0), "v"(s.n0), "v"(s.pm1), "v"(s.pcr), "v"(s.phm)
                          : "memory");
     }
-    /* uniform: every lane's mask field is LAND */
-    __device__ __forceinline__ bool all_land(int WalkLane<R>::*m)
+    /* uniform: the lanes whose mask field is not LAND / is LAND (a wavefront
+     * is all LAND when the first is 0, has no LAND lane when the second is);
+     * masks rather than bools keep the row-form bits in scalar registers */
+    __device__ __forceinline__ uint64_t non_land_lanes(int WalkLane<R>::*m)
     {
-        return __builtin_amdgcn_ballot_w64(s.*m != FCA_LAND) == 0;
+        return __builtin_amdgcn_ballot_w64(s.*m != FCA_LAND);
     }
-    /* uniform: no lane's mask field is LAND */
-    __device__ __forceinline__ bool no_land(int WalkLane<R>::*m)
+    __device__ __forceinline__ uint64_t land_lanes(int WalkLane<R>::*m)
     {
-        return __builtin_amdgcn_ballot_w64(s.*m == FCA_LAND) == 0;
+        return __builtin_amdgcn_ballot_w64(s.*m == FCA_LAND);
     }
     /* uniform: some lane's flag is set */
     __device__ __forceinline__ bool any(int WalkLane<R>::*f)

@@ -824,8 +824,11 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         }
         /* the row forms from row r's mask, fetched two rows ago, so that no
          * load of this row is waited for here */
-        land_rows = (land_rows << 1) | (lp.all_land(&L::m0) ? 1u : 0u);
-        void_rows = (void_rows << 1) | (lp.no_land(&L::m0) ? 1u : 0u);
+        {
+            const uint64_t nonland = lp.non_land_lanes(&L::m0), land = lp.land_lanes(&L::m0);
+            land_rows = (land_rows << 1) | (nonland == 0 ? 1u : 0u);
+            void_rows = (void_rows << 1) | (land == 0 ? 1u : 0u);
+        }
         w.r = r;
         if (a.p.fast && cols_inner && (land_rows & 0x1Fu) == 0x1Fu && r - 4 >= 1 &&
             r <= nr - 2) {

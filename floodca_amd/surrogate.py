@@ -576,6 +576,21 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
     res["judged_rmse_uncertainty_m"] = rmse(se[na:nf])
     res["judged_rmse_audit_m"] = rmse(se[nf:]) if len(audit) else float("nan")
     res["judged_rmse_audit_95ci_m"] = list(rmse_interval(se[nf:], seed=pc.seed))
+    # the error of what the pipeline hands back, judged by the CA: the flagged
+    # members return the CA's own maps (no error), the unflagged ones the
+    # network's, whose error the audit measures. Over the judged members:
+    se_arr = np.asarray(se, dtype=np.float64)
+    w_all = max(float(se_arr[:, 1].sum()), 1e-30)
+    res["judged_delivered_rmse_m"] = (float(math.sqrt(se_arr[nf:, 0].sum() / w_all))
+                                      if len(audit) else float("nan"))
+    # and over all candidates, the audit standing in for the unflagged ones
+    if len(audit):
+        n_unfl = pc.n_cand - nf
+        w_aud = max(float(se_arr[nf:, 1].sum()), 1e-30)
+        w_unfl = w_aud * n_unfl / len(audit)
+        mse_aud = float(se_arr[nf:, 0].sum()) / w_aud
+        res["delivered_rmse_all_est_m"] = float(
+            math.sqrt(mse_aud * w_unfl / (w_unfl + float(se_arr[:nf, 1].sum()))))
     res["judged_rmse_depth_baseline_m"] = rmse(dep)
     res["judged_rmse_audit_depth_baseline_m"] = rmse(dep[nf:]) if len(audit) else float("nan")
     res["judged_mean_ca_max_depth_m"] = float(np.mean([r[1] for r in ca]))
@@ -612,7 +627,10 @@ def run_pipeline(base: Grid, pc: PipelineCfg = PipelineCfg(), log=print) -> dict
         f"{res['judged_rmse_audit_m'] * 100:.2f} cm (95% interval "
         f"{res['judged_rmse_audit_95ci_m'][0] * 100:.2f}-"
         f"{res['judged_rmse_audit_95ci_m'][1] * 100:.2f}; depth baseline "
-        f"{res['judged_rmse_audit_depth_baseline_m'] * 100:.2f} cm)")
+        f"{res['judged_rmse_audit_depth_baseline_m'] * 100:.2f} cm); delivered (CA maps on "
+        f"the flagged, the network's on the rest) {res['judged_delivered_rmse_m'] * 100:.2f} cm "
+        f"on the judged, {res.get('delivered_rmse_all_est_m', float('nan')) * 100:.2f} cm "
+        f"estimated over all candidates")
     log(f"surrogate: {res['scenarios_per_h']:.0f} "
         f"scenarios/h vs all-CA alone {res['all_ca_alone_scenarios_per_h']:.0f} "
         f"({res['speedup_vs_alone']:.1f}x; {res['speedup_vs_alone_with_training']:.1f}x "

@@ -54,23 +54,21 @@ struct CpuLanes {
         form_rows[form]++;
     }
     void loads_settled() {}
-    bool no_land(int WalkLane<R>::*m)
+    uint64_t land_lanes(int WalkLane<R>::*m)
     {
+        uint64_t b = 0;
         for (int l = 0; l < FCA_WAVE; l++) {
-            if (s[l].*m == FCA_LAND) {
-                return false;
-            }
+            b |= (uint64_t)(s[l].*m == FCA_LAND) << l;
         }
-        return true;
+        return b;
     }
-    bool all_land(int WalkLane<R>::*m)
+    uint64_t non_land_lanes(int WalkLane<R>::*m)
     {
+        uint64_t b = 0;
         for (int l = 0; l < FCA_WAVE; l++) {
-            if (s[l].*m != FCA_LAND) {
-                return false;
-            }
+            b |= (uint64_t)(s[l].*m != FCA_LAND) << l;
         }
-        return true;
+        return b;
     }
     bool any(int WalkLane<R>::*f)
     {

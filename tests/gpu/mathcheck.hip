@@ -78,13 +78,13 @@ __global__ void k_eval(int kind, uint32_t first, uint32_t count, McArgs a, float
     case 16: y = div_step_fast(x, a.d, a.rd, step_den_fast(a.d)); break;
 #if defined(__HIP_DEVICE_COMPILE__)
     case 17: { /* certified critical speed; -1 where it defers to crit_speed */
-        int fix = 0;
+        bool fix = false;
         const float v = crit_speed_cert(a.g, x, fix);
         y = fix ? -1.0f : v;
         break;
     }
     case 18: { /* certified cube root; -1 where it defers */
-        int fix = 0;
+        bool fix = false;
         const float v = cbrt_sq_cert(x, fix);
         y = fix ? -1.0f : v;
         break;
