@@ -243,16 +243,16 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
     dbase = E.device_base(base, prec)
     cells = sum(s.side * s.side for s in scen)
     t_base = time.time() - t0
-    (strip, tpw), tune = E.autotune_tile_crops(dbase, scen, base.dx, prec)
+    (strip, tpw, cpl), tune = E.autotune_tile_crops(dbase, scen, base.dx, prec)
     ph = {}
     t_b, r_b = E.run_batched_crops(dbase, scen, base.dx, prec, strip_rows=strip, phases=ph,
-                                   tiles_per_wave=tpw)
+                                   tiles_per_wave=tpw, cols_per_lane=cpl)
     rng = np.random.default_rng(seed + 1)
     idx = sorted(rng.choice(n_scen, size=min(n_alone, n_scen), replace=False).tolist())
     sub = [scen[i] for i in idx]
     t_a, r_a = E.run_alone_crops(dbase, sub, base.dx, prec)
     t_s, r_s = E.run_batched_crops(dbase, sub, base.dx, prec, strip_rows=strip,
-                                   tiles_per_wave=tpw)
+                                   tiles_per_wave=tpw, cols_per_lane=cpl)
     dbase.close()
     same = r_a == r_s == [r_b[i] for i in idx]
     steps = [r[2] for r in r_b]
@@ -260,7 +260,8 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
                                  == A.FCA_LAND)) for sc in scen]
     land_updates = float(sum(n * k for n, k in zip(land, steps)))
     print(f"[bench] ensemble: {n_scen} scenarios ({cells / 1e6:.0f} M cells), base on the "
-          f"device in {t_base:.1f}s, strip {strip} x {tpw} tiles per wavefront "
+          f"device in {t_base:.1f}s, strip {strip} x {tpw} tiles per wavefront x {cpl} "
+          f"columns per work-item "
           f"({ {k: round(v, 3) for k, v in tune.items()} }), "
           f"batched {t_b:.2f}s ({ {k: round(v, 2) for k, v in ph.items()} }); "
           f"sample of {len(idx)}: alone {t_a:.2f}s, "
@@ -279,7 +280,7 @@ def ensemble_bench(base, prec: str, n_scen: int, n_alone: int, seed: int = 20261
         "land_cell_updates_per_s_batched_run": land_updates / ph["run"],
         "scenarios": n_scen, "sample": len(idx), "cells": cells,
         "steps_min_max": [min(steps), max(steps)], "strip_rows": strip,
-        "tiles_per_wave": tpw,
+        "tiles_per_wave": tpw, "cols_per_lane": cpl,
         "autotune_s": {k: round(v, 4) for k, v in tune.items()},
         "workload": "crops of the 16384^2 DEM, side U[256, 2048], built on the device; "
                     "triangular storm D U[2, 12] h, 50 mm/h x lognormal(0, 0.5); Manning x "
@@ -553,6 +554,8 @@ def main() -> int:
     ap.add_argument("--prec", choices=("f32", "f64"), default="f32")
     ap.add_argument("--kernel", choices=("fused", "split"), default="fused")
     ap.add_argument("--strip", type=int, default=0, help="rows per wavefront strip (0 = default)")
+    ap.add_argument("--cols-per-lane", type=int, default=0, choices=(0, 1, 2),
+                    help="columns each work-item walks (0 = default 1; 2: float only)")
     ap.add_argument("--neigh", type=int, default=4, choices=(4, 8))
     ap.add_argument("--no-land-form", action="store_true",
                     help="fused walk in its general form only (same results)")
@@ -614,7 +617,8 @@ def main() -> int:
 
     timer = HipTimer()
     flags = A.FCA_HIP_NO_LAND_FORM if args.no_land_form else 0
-    with Sim(g, cfg, acc, prec=prec, kernel=kern, strip_rows=args.strip, flags=flags) as sim:
+    with Sim(g, cfg, acc, prec=prec, kernel=kern, strip_rows=args.strip, flags=flags,
+             cols_per_lane=args.cols_per_lane) as sim:
         stream = C.c_void_p(sim.stream)
         sim.advance(args.spinup + args.warmup)
         sim.sync()
@@ -635,7 +639,8 @@ def main() -> int:
     achieved = bps / (ms_per_step * 1e-3) / 1e9
     workload = f"fused-{prec}-K{args.neigh}-{args.size}"
     # measured for the default kernel options only
-    default_kernel = kern == A.FCA_HIP_FUSED and args.strip == 0 and not args.no_land_form
+    default_kernel = (kern == A.FCA_HIP_FUSED and args.strip == 0 and not args.no_land_form
+                      and args.cols_per_lane in (0, 1))
     traffic = load_traffic(workload) if default_kernel else None
     valu = load_valu(workload) if default_kernel else None
     emass = abs(out.V + out.V_sea + out.V_bnd - out.V_rain - out.V0 - out.clamp_err) / \
@@ -664,6 +669,7 @@ def main() -> int:
             "timed_steps": [first, first - 1 + args.steps],
             "grid": [args.size, args.size], "land_cells": n_land, "all_cells": n_all,
             "kernel": args.kernel, "neigh": args.neigh, "sim_time_s": out.t,
+            "cols_per_lane": args.cols_per_lane or 1,
             "all_cell_updates_per_s": n_all * args.steps / (ms * 1e-3),
             "mass_err_rel": emass,
         },

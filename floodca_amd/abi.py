@@ -82,7 +82,7 @@ def _make(real):
     class FcaHipOpts(C.Structure):
         _fields_ = [("kernel", C.c_int), ("strip_rows", C.c_int), ("graph_batch", C.c_int),
                     ("record_steps", C.c_int), ("max_blocks", C.c_int),
-                    ("flags", C.c_int), ("tiles_per_wave", C.c_int), ("reserved", C.c_int)]
+                    ("flags", C.c_int), ("tiles_per_wave", C.c_int), ("cols_per_lane", C.c_int)]
 
     class FcaHipStepRec(C.Structure):
         _fields_ = [("t", C.c_double), ("F", C.c_double), ("dt_base", real), ("dt", real),

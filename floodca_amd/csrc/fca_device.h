@@ -208,6 +208,53 @@ FCA_HDI double fabs_r(double x) { return __builtin_fabs(x); }
 FCA_HDI float fmax_r(float a, float b) { return __builtin_fmaxf(a, b); }
 FCA_HDI double fmax_r(double a, double b) { return __builtin_fmax(a, b); }
 
+#if defined(__HIPCC__)
+/* A DPP move with control CTRL on rows ROW_MASK; lanes of other rows, and
+ * lanes whose source is outside their row, read +0. */
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_or_zero(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROW_MASK, 0xf, true));
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_or_zero(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffll), CTRL, ROW_MASK,
+                                               0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float lane63(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double lane63(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+/* uniform: the largest v over the wavefront, for v >= +0 and not NaN on
+ * every lane, with every lane active. A max-scan within each row of 16 lanes
+ * (shifts of 1, 2, 4, 8), then lane 15 of rows 0 and 2 into rows 1 and 3 and
+ * lane 31 into rows 2 and 3 leave the maximum in lane 63. Lanes with no
+ * source read +0, the identity of this max. */
+template <class R>
+__device__ inline R wave_max_nonneg(R v)
+{
+    v = fmax_r(v, dpp_or_zero<0x111, 0xf>(v)); /* row_shr:1 */
+    v = fmax_r(v, dpp_or_zero<0x112, 0xf>(v)); /* row_shr:2 */
+    v = fmax_r(v, dpp_or_zero<0x114, 0xf>(v)); /* row_shr:4 */
+    v = fmax_r(v, dpp_or_zero<0x118, 0xf>(v)); /* row_shr:8 */
+    v = fmax_r(v, dpp_or_zero<0x142, 0xa>(v)); /* row_bcast:15 */
+    v = fmax_r(v, dpp_or_zero<0x143, 0xc>(v)); /* row_bcast:31 */
+    return lane63(v);
+}
+#endif
+
 /* x if x > 0, else +0: the term an outflow sum adds for x. */
 template <class R>
 FCA_HDI R pos_part(R x)

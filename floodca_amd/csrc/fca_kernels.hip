@@ -25,7 +25,8 @@
 #include "fca_launch.h"
 #include "fca_walk.h"
 
-static_assert(FCA_TILE_COLS == FCA_OUTC, "host tile width differs from the walk's");
+static_assert(FCA_TILE_COLS(1) == FCA_TILE_OUT(64) && FCA_TILE_COLS(2) == FCA_TILE_OUT(128),
+              "host tile width differs from the walk's");
 
 typedef real R;
 typedef CandKey<R>::type Key;
@@ -70,6 +71,7 @@ __device__ __forceinline__ double lane_from_left(double x)
 
 /* Lane policy of the walk on the GPU: one work-item is one lane. */
 struct GpuLanes {
+    static constexpr int WIDTH = 64;
     WalkLane<R> s;
     int lane;
     template <class F>
@@ -113,18 +115,18 @@ struct GpuLanes {
     {
         return __builtin_amdgcn_ballot_w64(s.*f != 0) != 0;
     }
-    /* every lane's dst = the largest src over the wavefront; every lane holds
-     * the same value after the butterfly, so readfirstlane is exact and lets
-     * the compiler keep dst in a scalar register */
+    /* uniform: the largest src over the wavefront (src >= +0, not NaN) */
     __device__ __forceinline__ void wave_max_to(R WalkLane<R>::*dst, R WalkLane<R>::*src)
     {
-        R v = s.*src;
-        for (int off = 32; off > 0; off >>= 1) {
-            const R o = __shfl_xor(v, off, 64);
-            v = (o > v) ? o : v;
-        }
-        s.*dst = uniform(v);
+        s.*dst = wave_max_nonneg(s.*src);
     }
+    /* the work-item's share of a sum / minimum over lanes */
+    template <class T>
+    __device__ __forceinline__ T item_sum(T WalkLane<R>::*f)
+    {
+        return s.*f;
+    }
+    __device__ __forceinline__ R item_min(R WalkLane<R>::*f) { return s.*f; }
     __device__ __forceinline__ static float uniform(float v)
     {
         return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
@@ -136,6 +138,89 @@ struct GpuLanes {
         const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
         return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
     }
+};
+
+/* Lane policy with two adjacent columns per work-item: a wavefront of 64
+ * work-items holds 128 lanes, lane 2i+k in s[k] of work-item i. A pull moves
+ * one lane: within the work-item, or across one DPP shift between s[1] of a
+ * work-item and s[0] of its neighbour. Ballots merge the two halves. The walk
+ * computes the same expressions on every lane as with one column per
+ * work-item; only its tiles are wider (122 written columns, against 58). */
+struct GpuLanes2 {
+    static constexpr int WIDTH = 128;
+    WalkLane<R> s[2];
+    int lane;
+    template <class F>
+    __device__ __forceinline__ void each(F f)
+    {
+        f(s[0], 2 * lane);
+        f(s[1], 2 * lane + 1);
+    }
+    template <class T>
+    __device__ __forceinline__ void pull_right(T WalkLane<R>::*dst, T WalkLane<R>::*src)
+    {
+        const T r = lane_from_right(s[0].*src);
+        s[0].*dst = s[1].*src;
+        s[1].*dst = r;
+    }
+    template <class T>
+    __device__ __forceinline__ void pull_left(T WalkLane<R>::*dst, T WalkLane<R>::*src)
+    {
+        const T l = lane_from_left(s[1].*src);
+        s[1].*dst = s[0].*src;
+        s[0].*dst = l;
+    }
+    __device__ __forceinline__ void count_form(int) {}
+    /* as GpuLanes::loads_settled, for both columns */
+    __device__ __forceinline__ void loads_settled()
+    {
+        __asm__ volatile("" ::"v"(s[0].lv0), "v"(s[0].n0), "v"(s[0].pm1), "v"(s[0].pcr),
+                         "v"(s[0].phm), "v"(s[1].lv0), "v"(s[1].n0), "v"(s[1].pm1),
+                         "v"(s[1].pcr), "v"(s[1].phm)
+                         : "memory");
+    }
+    /* uniform: nonzero when some lane's mask field is not LAND / is LAND (the
+     * work-items whose column 0 or 1 qualifies) */
+    __device__ __forceinline__ uint64_t non_land_lanes(int WalkLane<R>::*m)
+    {
+        return __builtin_amdgcn_ballot_w64(s[0].*m != FCA_LAND) |
+               __builtin_amdgcn_ballot_w64(s[1].*m != FCA_LAND);
+    }
+    __device__ __forceinline__ uint64_t land_lanes(int WalkLane<R>::*m)
+    {
+        return __builtin_amdgcn_ballot_w64(s[0].*m == FCA_LAND) |
+               __builtin_amdgcn_ballot_w64(s[1].*m == FCA_LAND);
+    }
+    __device__ __forceinline__ bool any(int WalkLane<R>::*f)
+    {
+        return (__builtin_amdgcn_ballot_w64(s[0].*f != 0) |
+                __builtin_amdgcn_ballot_w64(s[1].*f != 0)) != 0;
+    }
+    __device__ __forceinline__ void wave_max_to(R WalkLane<R>::*dst, R WalkLane<R>::*src)
+    {
+        const R m = wave_max_nonneg(fmax_r(s[0].*src, s[1].*src));
+        s[0].*dst = m;
+        s[1].*dst = m;
+    }
+    template <class T>
+    __device__ __forceinline__ T item_sum(T WalkLane<R>::*f)
+    {
+        return s[0].*f + s[1].*f;
+    }
+    __device__ __forceinline__ R item_min(R WalkLane<R>::*f)
+    {
+        return (s[1].*f < s[0].*f) ? s[1].*f : s[0].*f;
+    }
+};
+
+/* The lane policy of CPL columns per work-item. */
+template <int CPL>
+struct LanesOf {
+    typedef GpuLanes type;
+};
+template <>
+struct LanesOf<2> {
+    typedef GpuLanes2 type;
 };
 
 /* ---- reductions ----------------------------------------------------------- */
@@ -249,10 +334,12 @@ struct FusedArgs {
 };
 
 /* One workgroup of the fused step: wg of nwg workgroups walking a.ntiles
- * tiles of one grid. */
-template <int K8, int QT>
+ * tiles of one grid, CPL columns per work-item. */
+template <int K8, int QT, int CPL>
 __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
 {
+    typedef typename LanesOf<CPL>::type Lanes;
+    const int outc = FCA_TILE_OUT(Lanes::WIDTH);
     __shared__ StepVals<R> sh_sv;
     __shared__ int sh_hbuf;
     __shared__ double sh_acc[FCA_BLOCK / 64][4];
@@ -275,7 +362,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
         return;
     }
 
-    GpuLanes lp;
+    Lanes lp;
     lp.lane = lane;
     walk_acc_init<R>(lp);
     {
@@ -304,7 +391,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
                 }
                 continue;
             }
-            walk_strip<R, K8, QT>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+            walk_strip<R, K8, QT>(lp, wa, r0, r1, grp * outc - FCA_HALO);
             if (QT) {
                 const bool wet = lp.any(&WalkLane<R>::qwet), chg = lp.any(&WalkLane<R>::qchg);
                 if (lane == 0) {
@@ -316,10 +403,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
     walk_acc_finish<R>(lp, a.p);
 
     {
-        double vr = wave_sum(lp.s.v_rain), cl = wave_sum(lp.s.clamp);
-        double vs = wave_sum(lp.s.v_sea), vb = wave_sum(lp.s.v_bnd);
-        long long cp = wave_sum(lp.s.cap);
-        R dm = wave_min(lp.s.dtc);
+        typedef WalkLane<R> L;
+        double vr = wave_sum(lp.item_sum(&L::v_rain)), cl = wave_sum(lp.item_sum(&L::clamp));
+        double vs = wave_sum(lp.item_sum(&L::v_sea)), vb = wave_sum(lp.item_sum(&L::v_bnd));
+        long long cp = wave_sum(lp.item_sum(&L::cap));
+        R dm = wave_min(lp.item_min(&L::dtc));
         if (lane == 0) {
             sh_acc[wave][0] = vr, sh_acc[wave][1] = cl, sh_acc[wave][2] = vs;
             sh_acc[wave][3] = vb, sh_cap[wave] = cp, sh_dtc[wave] = dm;
@@ -343,16 +431,31 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
     }
 }
 
-/* Wavefronts per SIMD the walk is compiled for. The K=4 walk without the
- * quiet-tile bookkeeping fits four (at most 128 VGPRs, no spills), whose
+/* Wavefronts per SIMD the walk is compiled for, each the most that compiles
+ * without spilling to scratch. Float, one column per work-item: the K=4 walk
+ * without the quiet-tile bookkeeping fits four (at most 128 VGPRs), whose
  * fourth wavefront hides row-load latency; the others need more registers
- * than that and run three. */
-#define FCA_FUSED_OCCUPANCY(K8, QT) __attribute__((amdgpu_waves_per_eu((K8) || (QT) ? 3 : 4)))
+ * and run three. Float, two columns per work-item: about 170 VGPRs for K=4
+ * (three wavefronts, FCA_CPL2_WAVES) and more with K=8 or the bookkeeping
+ * (two). Double, one column per work-item: three for K=4, two for K=8 (its
+ * two-column form is not built). */
+#ifndef FCA_CPL2_WAVES
+#define FCA_CPL2_WAVES 3
+#endif
+#ifdef FCA_DOUBLE
+#define FCA_FUSED_WAVES(K8, QT, CPL) ((K8) ? 2 : 3)
+#else
+#define FCA_FUSED_WAVES(K8, QT, CPL) \
+    ((CPL) == 2 ? ((K8) || (QT) ? 2 : FCA_CPL2_WAVES) : ((K8) || (QT) ? 3 : 4))
+#endif
+#define FCA_FUSED_OCCUPANCY(K8, QT, CPL) \
+    __attribute__((amdgpu_waves_per_eu(FCA_FUSED_WAVES(K8, QT, CPL))))
 
-template <int K8, int QT>
-__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_OCCUPANCY(K8, QT) k_fused(FusedArgs a)
+template <int K8, int QT, int CPL>
+__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_OCCUPANCY(K8, QT, CPL)
+    k_fused(FusedArgs a)
 {
-    fused_body<K8, QT>(a, blockIdx.x, gridDim.x);
+    fused_body<K8, QT, CPL>(a, blockIdx.x, gridDim.x);
 }
 
 /* ---- batched fused step ------------------------------------------------------
@@ -369,8 +472,8 @@ struct ScenDev {
     int rec_cap, nblocks;
 };
 
-template <int K8>
-__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_OCCUPANCY(K8, 0)
+template <int K8, int CPL>
+__global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_OCCUPANCY(K8, 0, CPL)
     k_fused_batch(const ScenDev *scen, const int2 *wgs, long long k)
 {
     const int2 e = wgs[blockIdx.x];
@@ -383,7 +486,7 @@ __global__ void __launch_bounds__(FCA_BLOCK) FCA_FUSED_OCCUPANCY(K8, 0)
     a.io.cand_reset = S->cand + (int)((k + 2) % 3);
     a.io.rec = S->rec;
     a.io.rec_cap = S->rec_cap;
-    fused_body<K8, 0>(a, e.y, S->nblocks);
+    fused_body<K8, 0, CPL>(a, e.y, S->nblocks);
 }
 
 /* Adds a block's balance terms to the block's own slot (a block always
@@ -668,22 +771,24 @@ __global__ void __launch_bounds__(FCA_BLOCK)
 
 __global__ void k_store_key(Key *dst, Key v) { *dst = v; }
 
-/* Per tile: bit 0 when none of its LAND output cells has cr != 0, bit 1 when
- * it has LAND output cells. One wavefront per tile. */
+/* Per tile (outc output columns): bit 0 when none of its LAND output cells
+ * has cr != 0, bit 1 when it has LAND output cells. One wavefront per tile. */
 __global__ void __launch_bounds__(64)
     k_tile_norain(const R *cr, const uint8_t *mask, int nr, int nc, int strip_rows, int ngroups,
-                  uint8_t *out)
+                  int outc, uint8_t *out)
 {
     const int t = blockIdx.x, lane = threadIdx.x;
     const int strip = t / ngroups, grp = t - strip * ngroups;
     const int r0 = strip * strip_rows, r1 = (r0 + strip_rows < nr) ? r0 + strip_rows : nr;
-    const int c = grp * FCA_OUTC + lane;
     int rain = 0, land = 0;
-    if (lane < FCA_OUTC && c < nc) {
-        for (int r = r0; r < r1; r++) {
-            const size_t i = (size_t)r * nc + c;
-            land |= mask[i] == FCA_LAND;
-            rain |= mask[i] == FCA_LAND && !(cr[i] == (R)0);
+    for (int cc = lane; cc < outc; cc += 64) {
+        const int c = grp * outc + cc;
+        if (c < nc) {
+            for (int r = r0; r < r1; r++) {
+                const size_t i = (size_t)r * nc + c;
+                land |= mask[i] == FCA_LAND;
+                rain |= mask[i] == FCA_LAND && !(cr[i] == (R)0);
+            }
         }
     }
     rain = __builtin_amdgcn_ballot_w64(rain) != 0;
@@ -943,7 +1048,8 @@ extern "C" void fca_k_geometry(FcaDev *d)
     if (d->strip_rows <= 0) {
         d->strip_rows = 256;
     }
-    d->ngroups = (d->nc + FCA_OUTC - 1) / FCA_OUTC;
+    const int outc = FCA_TILE_COLS(d->cols_per_lane == 2 ? 2 : 1);
+    d->ngroups = (d->nc + outc - 1) / outc;
     d->nstrips = (d->nr + d->strip_rows - 1) / d->strip_rows;
     d->ntiles = d->ngroups * d->nstrips;
     d->nblocks = (d->ntiles + FCA_BLOCK / 64 - 1) / (FCA_BLOCK / 64);
@@ -1004,17 +1110,25 @@ extern "C" int fca_k_step(const FcaDev *d, long long k, hipStream_t st)
         FusedArgs a = fused_args(d);
         a.io = ctl_io(d, k);
         const int qt = a.skip;
+        const dim3 g(d->nblocks), b(FCA_BLOCK);
+#define FCA_LAUNCH_FUSED(K8_, QT_)                                                 \
+    if (FCA_CPL2_BUILT && d->cols_per_lane == 2) {                                 \
+        hipLaunchKernelGGL((k_fused<K8_, QT_, FCA_CPL2_BUILT ? 2 : 1>), g, b, 0, st, a); \
+    } else {                                                                       \
+        hipLaunchKernelGGL((k_fused<K8_, QT_, 1>), g, b, 0, st, a);                \
+    }
         if (d->K == 8) {
             if (qt) {
-                hipLaunchKernelGGL((k_fused<1, 1>), dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
+                FCA_LAUNCH_FUSED(1, 1)
             } else {
-                hipLaunchKernelGGL((k_fused<1, 0>), dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
+                FCA_LAUNCH_FUSED(1, 0)
             }
         } else if (qt) {
-            hipLaunchKernelGGL((k_fused<0, 1>), dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
+            FCA_LAUNCH_FUSED(0, 1)
         } else {
-            hipLaunchKernelGGL((k_fused<0, 0>), dim3(d->nblocks), dim3(FCA_BLOCK), 0, st, a);
+            FCA_LAUNCH_FUSED(0, 0)
         }
+#undef FCA_LAUNCH_FUSED
     } else {
         SplitArgs a;
         const size_t N = (size_t)d->nr * d->nc;
@@ -1070,18 +1184,25 @@ extern "C" void *fca_k_batch_table(const FcaDev *const *devs, int n)
     return dv;
 }
 
-extern "C" int fca_k_step_batch(const void *tab, const void *wgs, int nwg, int K, long long k,
-                                hipStream_t st)
+extern "C" int fca_k_step_batch(const void *tab, const void *wgs, int nwg, int K, int cpl,
+                                long long k, hipStream_t st)
 {
     if (nwg <= 0) {
         return 0;
     }
+    const ScenDev *sc = (const ScenDev *)tab;
+    const int2 *w = (const int2 *)wgs;
+    const int c2 = FCA_CPL2_BUILT ? 2 : 1;
     if (K == 8) {
-        hipLaunchKernelGGL(k_fused_batch<1>, dim3(nwg), dim3(FCA_BLOCK), 0, st,
-                           (const ScenDev *)tab, (const int2 *)wgs, k);
+        if (FCA_CPL2_BUILT && cpl == 2) {
+            hipLaunchKernelGGL((k_fused_batch<1, c2>), dim3(nwg), dim3(FCA_BLOCK), 0, st, sc, w, k);
+        } else {
+            hipLaunchKernelGGL((k_fused_batch<1, 1>), dim3(nwg), dim3(FCA_BLOCK), 0, st, sc, w, k);
+        }
+    } else if (FCA_CPL2_BUILT && cpl == 2) {
+        hipLaunchKernelGGL((k_fused_batch<0, c2>), dim3(nwg), dim3(FCA_BLOCK), 0, st, sc, w, k);
     } else {
-        hipLaunchKernelGGL(k_fused_batch<0>, dim3(nwg), dim3(FCA_BLOCK), 0, st,
-                           (const ScenDev *)tab, (const int2 *)wgs, k);
+        hipLaunchKernelGGL((k_fused_batch<0, 1>), dim3(nwg), dim3(FCA_BLOCK), 0, st, sc, w, k);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -1092,7 +1213,8 @@ extern "C" int fca_k_tile_norain(const FcaDev *d, hipStream_t st)
         return 0;
     }
     hipLaunchKernelGGL(k_tile_norain, dim3(d->ntiles), dim3(64), 0, st, d->cr, d->mask, d->nr,
-                       d->nc, d->strip_rows, d->ngroups, d->tnorain);
+                       d->nc, d->strip_rows, d->ngroups,
+                       FCA_TILE_COLS(d->cols_per_lane == 2 ? 2 : 1), d->tnorain);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

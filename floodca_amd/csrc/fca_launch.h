@@ -13,10 +13,18 @@
 extern "C" {
 #endif
 
+/* Output columns of one wavefront tile with cpl (1 or 2) columns per
+ * work-item: 64*cpl lanes less 3 halo lanes a side. */
+#define FCA_TILE_COLS(cpl) (64 * (cpl) - 6)
+/* Whether this precision's library has the two-column walk (float only: in
+ * double it would spill registers to scratch). */
+#ifdef FCA_DOUBLE
+#define FCA_CPL2_BUILT 0
+#else
+#define FCA_CPL2_BUILT 1
+#endif
+
 /* Device-resident state of one grid. */
-/* Output columns of one wavefront tile: 64 lanes less 3 halo lanes a side
- * (FCA_OUTC of the walk). */
-#define FCA_TILE_COLS 58
 
 typedef struct FcaDev {
     int nr, nc, K, kernel;
@@ -40,8 +48,10 @@ typedef struct FcaDev {
      * division flags it relies on) */
     int land_form;
     /* fused geometry; max_blocks caps the persistent grid (0 = one wave per tile),
-     * tiles_per_wave > 1 sizes it to walk that many tiles per wavefront */
+     * tiles_per_wave > 1 sizes it to walk that many tiles per wavefront,
+     * cols_per_lane (1 or 2) sets the tile width, FCA_TILE_COLS(cols_per_lane) */
     int strip_rows, ngroups, nstrips, ntiles, nblocks, max_blocks, tiles_per_wave;
+    int cols_per_lane;
     /* quiet-tile skipping: flags [2][ntiles], rain-free tiles [ntiles], on/off */
     uint8_t *tflags, *tnorain;
     int skip;
@@ -78,8 +88,9 @@ int fca_k_grid_scan(const FcaDev *d, unsigned long long *out, hipStream_t st);
 void *fca_k_batch_table(const FcaDev *const *devs, int n);
 /* One batched launch: nwg workgroups listed in wgs (device array of int2
  * {grid index in the table, workgroup within that grid}); launch index k
- * selects every grid's controller parity and candidate slots. */
-int fca_k_step_batch(const void *tab, const void *wgs, int nwg, int K, long long k,
+ * selects every grid's controller parity and candidate slots; every grid of
+ * the table has cpl columns per work-item. */
+int fca_k_step_batch(const void *tab, const void *wgs, int nwg, int K, int cpl, long long k,
                      hipStream_t st);
 /* Fills the fused geometry of d from d->strip_rows. */
 void fca_k_geometry(FcaDev *d);

@@ -1,9 +1,10 @@
 /*
  * fca_walk.h -- the fused WCA step as a column walk.
  *
- * One 64-lane wavefront owns 64 adjacent columns and walks down a strip of
- * rows. Lane l holds column c = c_first + l; the three lanes at each side are
- * halo, so a wavefront writes 58 columns. At row r the walk
+ * One wavefront owns W adjacent columns (W = 64, one column per work-item,
+ * or 128, two adjacent columns per work-item) and walks down a strip of rows.
+ * Lane l (0..W-1) holds column c = c_first + l; the three lanes at each side
+ * are halo, so a wavefront writes W-6 columns. At row r the walk
  *   A. reads row r (z, h, n, mask) and forms the faces that touch it: the
  *      east face of row r, the face between rows r-1 and r and the two K=8
  *      diagonals between them (physics.h:51-125, model.c:393-482);
@@ -17,7 +18,7 @@
  * Rows r-1..r-4 stay in a register ring, so every field crosses HBM once per
  * step; horizontal neighbours come from the adjacent lanes.
  *
- * Two forms of a row. Where rows r-4..r are LAND on all 64 lanes, away from
+ * Two forms of a row. Where rows r-4..r are LAND on all W lanes, away from
  * the grid's edges, a row takes the LAND form: no wall, sea or open-boundary
  * terms, levels formed once per cell. Any other row takes the general form,
  * which keeps every rule. Both evaluate the same expressions in the same
@@ -28,10 +29,11 @@
  * step's minimum time step (alpha*dx / v is non-increasing in v), so its
  * exact speed is only evaluated when a cheap upper bound exceeds vthr.
  *
- * The walk is a template over a lane policy LP: on the GPU a lane is one
- * work-item and pull_left/pull_right are DPP wavefront shifts; the CPU
- * emulation runs 64 lanes in lockstep with array shifts, so the CPU test
- * suite checks the same dataflow bit for bit against the oracle.
+ * The walk is a template over a lane policy LP of LP::WIDTH lanes: on the
+ * GPU a lane is one work-item (or one of a work-item's two columns) and
+ * pull_left/pull_right are DPP wavefront shifts; the CPU emulation runs the
+ * lanes in lockstep with array shifts, so the CPU test suite checks the same
+ * dataflow bit for bit against the oracle.
  *
  * hmax and tmax. The reference sets hmax = h and tmax = t + dt whenever a
  * step leaves h > hmax (model.c:585-590), which under steady rain is every
@@ -46,8 +48,8 @@
  * cells still rising (hmax = h, tmax = the last step's value) before any
  * read of hmax or tmax.
  *
- * Lane validity: faces are valid on lanes 0..62, limiters on 1..62, new
- * depths on 2..61 and time-step candidates on 3..60, which are the lanes a
+ * Lane validity: faces are valid on lanes 0..W-2, limiters on 1..W-2, new
+ * depths on 2..W-3 and time-step candidates on 3..W-4, which are the lanes a
  * wavefront writes. Values a lane pulls from outside the wavefront only
  * reach lanes that are not valid.
  */
@@ -58,7 +60,9 @@
 
 #define FCA_WAVE 64
 #define FCA_HALO 3
-#define FCA_OUTC (FCA_WAVE - 2 * FCA_HALO)
+/* columns a wavefront of `width` lanes writes */
+#define FCA_TILE_OUT(width) ((width) - 2 * FCA_HALO)
+#define FCA_OUTC FCA_TILE_OUT(FCA_WAVE)
 
 /* Inputs of one launch of the fused step. */
 template <class R>
@@ -710,7 +714,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
 }
 
 /* Walks one strip for one wavefront. strip covers output rows
- * [r_out0, r_out1) and columns [c_first + FCA_HALO, c_first + FCA_HALO + 58).
+ * [r_out0, r_out1) and columns [c_first + FCA_HALO, c_first + LP::WIDTH - FCA_HALO).
  * QT: also record whether the strip's cells end wet or changed (qwet, qchg)
  * for quiet-tile skipping. */
 template <class R, int K8, int QT, class LP>
@@ -721,18 +725,18 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
     const int rb = (r_out0 - 3 > 0) ? r_out0 - 3 : 0;
     const int re = (r_out1 + 2 < nr + 2) ? r_out1 + 2 : nr + 2; /* inclusive */
     /* every lane inside the grid and none on its first or last column */
-    const int cols_inner = c_first >= 1 && c_first + FCA_WAVE - 1 <= nc - 2;
+    const int cols_inner = c_first >= 1 && c_first + LP::WIDTH - 1 <= nc - 2;
     WalkRow w;
     unsigned land_rows; /* bit k: row r-k is LAND on every lane */
     unsigned void_rows; /* bit k: row r-k has no LAND lane */
     int r;
 
     w.r_out0 = r_out0, w.r_out1 = r_out1, w.c_first = c_first;
-    w.edge_cols = c_first <= 0 || c_first + FCA_WAVE - 1 >= nc - 1;
+    w.edge_cols = c_first <= 0 || c_first + LP::WIDTH - 1 >= nc - 1;
     lp.each([&](L &s, int lane) {
         s.c = c_first + lane;
         s.colok = s.c >= 0 && s.c < nc;
-        s.outcol = s.colok && lane >= FCA_HALO && lane < FCA_WAVE - FCA_HALO;
+        s.outcol = s.colok && lane >= FCA_HALO && lane < LP::WIDTH - FCA_HALO;
         s.boff = (uint32_t)s.c * (uint32_t)sizeof(R);
         s.z0 = s.z1 = s.z2 = s.z3 = s.z4 = (R)0;
         s.m0 = s.m1 = s.m2 = s.m3 = s.m4 = FCA_NONE;
@@ -766,9 +770,9 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
     land_rows = 0u;
     void_rows = ~0u;
 
-    for (r = rb; r <= re; r++) {
-        /* take row r (prefetched), prefetch z/h/n of row r+1 and the mask of
-         * row r+2 */
+    /* take row r (prefetched), prefetch z/h/n of row r+1 and the mask of row
+     * r+2, and form row r's form bits */
+    auto take_row = [&](int r) {
         /* Every memory access of a row is issued here, at the top: the
          * loads are consumed one row later, so each has a whole row of work
          * to arrive in, and the stores of row r-3 (computed one row ago)
@@ -830,27 +834,14 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             void_rows = (void_rows << 1) | (land == 0 ? 1u : 0u);
         }
         w.r = r;
-        if (a.p.fast && cols_inner && (land_rows & 0x1Fu) == 0x1Fu && r - 4 >= 1 &&
-            r <= nr - 2) {
-            lp.count_form(1);
-            walk_row<R, K8, 1, QT>(lp, a, w);
-        } else if ((void_rows & 0xFu) == 0xFu) {
-            /* rows r-3..r hold no LAND cell: every face touching rows r-1
-             * and r is dead, the limiter of row r-1 is 1, row r-2 keeps its
-             * depth and row r-3 has no time step (the general form's values
-             * for non-LAND cells) */
-            lp.count_form(2);
-            lp.each([&](L &s, int) {
-                s.qx0 = s.qy1 = s.qd1_1 = s.qd2_1 = (R)0;
-                s.s1 = (R)1;
-                s.hn2 = s.h2;
-            });
-        } else {
-            lp.count_form(0);
-            walk_row<R, K8, 0, QT>(lp, a, w);
-        }
-
-        /* ---- rotate the ring ----------------------------------------------- */
+    };
+    /* row r takes the LAND form: rows r-4..r LAND on every lane, away from
+     * the grid's edges */
+    auto land_form = [&](int r) {
+        return a.p.fast && cols_inner && (land_rows & 0x1Fu) == 0x1Fu && r - 4 >= 1 &&
+               r <= nr - 2;
+    };
+    auto rotate = [&]() {
         lp.each([&](L &s, int) {
             s.z4 = s.z3, s.z3 = s.z2, s.z2 = s.z1, s.z1 = s.z0;
             s.m4 = s.m3, s.m3 = s.m2, s.m2 = s.m1, s.m1 = s.m0;
@@ -865,6 +856,56 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             s.s3 = s.s2, s.s2 = s.s1;
             s.pcr = s.pcr1, s.phm = s.phm1;
         });
+    };
+    /* one LAND row of a run, then take the next row; false when the run
+     * ends (the walk's end, or the next row is not LAND form) */
+    auto land_step = [&](int &r) {
+        lp.count_form(1);
+        walk_row<R, K8, 1, QT>(lp, a, w);
+        rotate();
+        if (++r > re) {
+            return false;
+        }
+        take_row(r);
+        return land_form(r);
+    };
+
+    r = rb;
+    if (r <= re) {
+        take_row(r);
+    }
+    while (r <= re) {
+        /* row r is taken */
+        if (land_form(r)) {
+            /* A run of LAND rows walks two rows per trip: the ring's shifts
+             * are then renames within the trip, and only the values a trip
+             * leaves in the ring cross the loop's back edge, rather than a
+             * move per ring slot and row. */
+            while (land_step(r) && land_step(r)) {
+            }
+            if (r > re) {
+                break;
+            }
+        }
+        if ((void_rows & 0xFu) == 0xFu) {
+            /* rows r-3..r hold no LAND cell: every face touching rows r-1
+             * and r is dead, the limiter of row r-1 is 1, row r-2 keeps its
+             * depth and row r-3 has no time step (the general form's values
+             * for non-LAND cells) */
+            lp.count_form(2);
+            lp.each([&](L &s, int) {
+                s.qx0 = s.qy1 = s.qd1_1 = s.qd2_1 = (R)0;
+                s.s1 = (R)1;
+                s.hn2 = s.h2;
+            });
+        } else {
+            lp.count_form(0);
+            walk_row<R, K8, 0, QT>(lp, a, w);
+        }
+        rotate();
+        if (++r <= re) {
+            take_row(r);
+        }
     }
     /* the last row's update is never an output row (re = r_out1 + 2), so no
      * deferred store is left; flush one all the same if the bounds change */

@@ -261,9 +261,9 @@ static void set_div_flags(FcaDev *d, int n_ok, int no_land_form)
  * between 16 rows and 96 (where the 5 halo rows a strip re-walks cost 5%).
  * A 16384^2 grid gets 96 rows, 4096^2 gets 23 (measured at 4096^2: 0.295
  * ms/step at 16 or 24 rows, 0.418 at 96). */
-static int default_strip_rows(int nrows, int ncols)
+static int default_strip_rows(int nrows, int ncols, int cpl)
 {
-    const long groups = (ncols + FCA_TILE_COLS - 1) / FCA_TILE_COLS;
+    const long groups = (ncols + FCA_TILE_COLS(cpl) - 1) / FCA_TILE_COLS(cpl);
     const long rows = (long)nrows * groups / (3 * 4096);
     return (int)(rows < 16 ? 16 : (rows > 96 ? 96 : rows));
 }
@@ -291,8 +291,10 @@ static int sim_alloc(FcaHipSim *s, const FcaGrid *g, const FcaRunCfg *cfg,
     d->nc = g->ncols;
     d->K = cfg->phys.K;
     d->kernel = o ? o->kernel : FCA_HIP_FUSED;
-    d->strip_rows = (o && o->strip_rows > 0) ? o->strip_rows
-                                             : default_strip_rows(g->nrows, g->ncols);
+    d->cols_per_lane = (o && o->cols_per_lane == 2) ? 2 : 1;
+    d->strip_rows = (o && o->strip_rows > 0)
+                        ? o->strip_rows
+                        : default_strip_rows(g->nrows, g->ncols, d->cols_per_lane);
     d->rec_cap = (o && o->record_steps > 0) ? o->record_steps : 0;
     set_run_consts(d, cfg);
     s->no_land_form = o && (o->flags & FCA_HIP_NO_LAND_FORM);
@@ -478,6 +480,12 @@ static int check_cfg(const FcaGrid *g, const FcaRunCfg *cfg, const FcaHipOpts *o
     }
     if (o && o->strip_rows < 0) {
         return fail(-1, "strip_rows must be >= 0");
+    }
+    if (o && (o->cols_per_lane < 0 || o->cols_per_lane > 2)) {
+        return fail(-1, "cols_per_lane must be 0, 1 or 2, not %d", o->cols_per_lane);
+    }
+    if (o && o->cols_per_lane == 2 && !FCA_CPL2_BUILT) {
+        return fail(-1, "cols_per_lane 2 is built for the float library only");
     }
     if (cfg->n_force_days < 0 || cfg->n_force_days > FCA_MAX_FORCE_DAYS) {
         return fail(-1, "n_force_days out of range");
@@ -1103,7 +1111,8 @@ int fca_hip_run_noalloc(FcaGrid *g, FcaRunCfg *cfg, FcaAcc *acc, FcaTimers *tm, 
     d->nc = g->ncols;
     d->K = cfg->phys.K;
     d->kernel = FCA_HIP_FUSED;
-    d->strip_rows = default_strip_rows(g->nrows, g->ncols);
+    d->cols_per_lane = 1;
+    d->strip_rows = default_strip_rows(g->nrows, g->ncols, 1);
     set_run_consts(d, cfg);
     fca_k_geometry(d);
     d->nslots = d->nblocks;
@@ -1413,6 +1422,7 @@ void fca_hip_ens_close(FcaHipEns *e)
 
 struct FcaHipBatch {
     int n, K;
+    int cpl; /* columns per work-item of every scenario's walk */
     FcaHipSim **sims;
     hipStream_t st;
     FcaDevCtrl *ctrl_pool; /* device: 2 records per scenario, contiguous */
@@ -1433,7 +1443,7 @@ struct FcaHipBatch {
 static int launch_batch_step(void *ctx, long long k)
 {
     FcaHipBatch *b = (FcaHipBatch *)ctx;
-    return fca_k_step_batch(b->tab, b->wgs_dev, b->nwg, b->K, k, b->st);
+    return fca_k_step_batch(b->tab, b->wgs_dev, b->nwg, b->K, b->cpl, k, b->st);
 }
 
 /* Workgroup list of the active scenarios: {scenario, workgroup} pairs. */
@@ -1516,6 +1526,7 @@ static FcaHipBatch *batch_new(int n, int K, const FcaHipOpts *opts)
     }
     b->n = n;
     b->K = K;
+    b->cpl = (opts && opts->cols_per_lane == 2) ? 2 : 1;
     b->batch = (opts && opts->graph_batch > 0) ? opts->graph_batch : 30;
     b->sims = (FcaHipSim **)calloc((size_t)n, sizeof(FcaHipSim *));
     b->active = (char *)calloc((size_t)n, 1);

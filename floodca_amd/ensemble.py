@@ -117,7 +117,7 @@ def device_base(base: Grid, prec: str = "f32"):
 
 
 def run_batched_crops(dbase, scens, dx: float, prec="f32", strip_rows=0, phases=None,
-                      run_wall=None, t_sim_cap=None, tiles_per_wave=0):
+                      run_wall=None, t_sim_cap=None, tiles_per_wave=0, cols_per_lane=0):
     """One batch of crop scenarios built on the device from dbase (a
     DeviceBase): returns (seconds, per-scenario (flooded, max depth, steps)),
     the time covering the configs, the scenario table, the run, the
@@ -129,7 +129,7 @@ def run_batched_crops(dbase, scens, dx: float, prec="f32", strip_rows=0, phases=
     crops = [crop_of(s, prec) for s in scens]
     t_cfg = time.perf_counter() - t0
     secs, res = run_crop_batch(dbase, crops, cfgs, prec, strip_rows, phases, run_wall,
-                               tiles_per_wave)
+                               tiles_per_wave, cols_per_lane)
     return t_cfg + secs, res
 
 
@@ -207,7 +207,7 @@ def day_config(sc: DayScenario, dx: float, prec: str = "f32"):
 
 
 def run_crop_batch(dbase, crops, cfgs, prec="f32", strip_rows=0, phases=None, run_wall=None,
-                   tiles_per_wave=0):
+                   tiles_per_wave=0, cols_per_lane=0):
     """One batch of crops (dicts as crop_of gives) with their configs, built
     on the device from dbase: returns (seconds, per-scenario (flooded, max
     depth, steps)), the time covering the scenario table, the run, the
@@ -216,7 +216,7 @@ def run_crop_batch(dbase, crops, cfgs, prec="f32", strip_rows=0, phases=None, ru
     t0 = time.perf_counter()
     res = []
     b = Batch.from_crops(dbase, crops, cfgs, prec=prec, strip_rows=strip_rows,
-                         tiles_per_wave=tiles_per_wave)
+                         tiles_per_wave=tiles_per_wave, cols_per_lane=cols_per_lane)
     try:
         t1 = time.perf_counter()
         b.run()
@@ -340,12 +340,12 @@ def autotune_strip(cases, prec="f32", candidates=(32, 48, 64, 96, 128), sample=2
 
 
 def autotune_tile_crops(dbase, scens, dx: float, prec="f32", strips=(32, 48, 64, 96, 128),
-                        tiles=(1, 2, 4), sample=512, seed=0):
+                        tiles=(1, 2, 4), cols=(1, 2), sample=512, seed=0):
     """The tile shape and cells per lane of a batch of crop scenarios built on
-    the device: the strip height (rows a wavefront walks per tile: the tile is
-    58 output columns by that many rows) and the tiles each wavefront walks
-    per launch (k > 1 launches a k-th of the workgroups, every lane walking k
-    strips in turn). Each candidate runs the sample batched, every scenario
+    the device: the strip height (rows a wavefront walks per tile), the tiles
+    each wavefront walks per launch (k > 1 launches a k-th of the workgroups,
+    every lane walking k strips in turn) and the columns each work-item walks
+    (1: tiles 58 output columns wide; 2: 122 columns, float only). Each candidate runs the sample batched, every scenario
     cut to its first 2 h of simulated time (runs of 20 min took ~10 ms, too
     short to rank them); the heights first at one tile per wavefront, then
     the tile counts at the best height. The sample should be as large as the
@@ -353,18 +353,26 @@ def autotune_tile_crops(dbase, scens, dx: float, prec="f32", strips=(32, 48, 64,
     (fuller GPU) and tall ones (fewer halo rows), and a sample of 24 crops
     picked 32 rows where batches of 512 ran faster with 96. Results do not
     depend on the choice (the walk's arithmetic is per cell), only the time
-    does. Returns ((strip_rows, tiles_per_wave), {"strip x tiles": seconds})."""
+    does. The heights are timed at one tile and one column per work-item,
+    then the tile counts at the best height, then the column counts at the
+    best of those. Returns ((strip_rows, tiles_per_wave, cols_per_lane),
+    {"strip x tiles x cols": seconds})."""
     rng = np.random.default_rng(seed)
     idx = rng.choice(len(scens), size=min(sample, len(scens)), replace=False)
     sub = [scens[i] for i in idx]
     timings = {}
 
-    def timed(strip, k):
-        wall = []
-        run_batched_crops(dbase, sub, dx, prec, strip_rows=strip, run_wall=wall,
-                          t_sim_cap=7200.0, tiles_per_wave=k)
-        timings[f"{strip}x{k}"] = wall[0]
-        return wall[0]
-    best_strip = min(strips, key=lambda st: timed(st, 1))
-    best = min(tiles, key=lambda k: timings.get(f"{best_strip}x{k}") or timed(best_strip, k))
-    return (best_strip, best), timings
+    def timed(strip, k, c):
+        key = f"{strip}x{k}x{c}"
+        if key not in timings:
+            wall = []
+            run_batched_crops(dbase, sub, dx, prec, strip_rows=strip, run_wall=wall,
+                              t_sim_cap=7200.0, tiles_per_wave=k, cols_per_lane=c)
+            timings[key] = wall[0]
+        return timings[key]
+    if prec != "f32":
+        cols = (1,)
+    best_strip = min(strips, key=lambda st: timed(st, 1, 1))
+    best_k = min(tiles, key=lambda k: timed(best_strip, k, 1))
+    best_c = min(cols, key=lambda c: timed(best_strip, best_k, c))
+    return (best_strip, best_k, best_c), timings

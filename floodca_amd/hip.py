@@ -134,11 +134,12 @@ def backend(prec: str = "f32") -> Backend:
 
 
 def opts(prec="f32", kernel=FCA_HIP_FUSED, strip_rows=0, graph_batch=0, record_steps=0,
-         max_blocks=0, flags=0, tiles_per_wave=0):
+         max_blocks=0, flags=0, tiles_per_wave=0, cols_per_lane=0):
     o = abi(prec).FcaHipOpts()
     o.kernel, o.strip_rows, o.graph_batch, o.record_steps = (kernel, strip_rows, graph_batch,
                                                              record_steps)
     o.max_blocks, o.flags, o.tiles_per_wave = max_blocks, flags, tiles_per_wave
+    o.cols_per_lane = cols_per_lane
     return o
 
 

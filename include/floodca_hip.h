@@ -255,7 +255,10 @@ typedef struct FcaHipOpts {
     int tiles_per_wave; /* fused: tiles each wavefront walks per launch (default 1):
                          * k > 1 launches a k-th of the workgroups, each wavefront
                          * walking k strips in turn (more cells per lane) */
-    int reserved;
+    int cols_per_lane;  /* fused: adjacent columns each work-item walks, 1 or 2
+                         * (default 1): tiles of 58 or 122 written columns (the
+                         * same results; 2 shares each row's work over two
+                         * columns at fewer wavefronts per SIMD) */
 } FcaHipOpts;
 
 /* flags: FCA_HIP_NO_LAND_FORM as above; FCA_HIP_NO_SKIP walks every tile

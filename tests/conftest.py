@@ -86,11 +86,12 @@ def emu_lib(prec):
 
 
 def emu_run(grid, cfg, acc, prec, strip_rows=0, cap=100000, land_form=True, forms=None,
-            skip=True, skipped=None):
+            skip=True, skipped=None, width=64):
     """Runs the emulated fused walk; land_form=False keeps every row in the
-    general form, skip=False walks every tile. forms, if a list, receives the
-    rows walked in the general, the LAND and the no-LAND form; skipped, if a
-    list, the number of quiet tiles skipped."""
+    general form, skip=False walks every tile, width=128 emulates two columns
+    per work-item (tiles of 122 columns). forms, if a list, receives the rows
+    walked in the general, the LAND and the no-LAND form; skipped, if a list,
+    the number of quiet tiles skipped."""
     import ctypes as C
 
     a = A.abi(prec)
@@ -100,12 +101,14 @@ def emu_run(grid, cfg, acc, prec, strip_rows=0, cap=100000, land_form=True, form
     n = C.c_int()
     L.emu_set_land_form(-1 if land_form else 0)
     L.emu_set_skip(1 if skip else 0)
+    L.emu_set_width(width)
     try:
         assert L.emu_run(C.byref(g), C.byref(cfg), C.byref(acc), strip_rows, rec, cap,
                          C.byref(n)) == 0
     finally:
         L.emu_set_land_form(-1)
         L.emu_set_skip(1)
+        L.emu_set_width(64)
     if skipped is not None:
         L.emu_skipped_tiles.restype = C.c_long
         skipped[:] = [L.emu_skipped_tiles()]

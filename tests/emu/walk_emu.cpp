@@ -2,7 +2,7 @@
  * walk_emu.cpp -- host emulation of the fused step, for the CPU test suite
  * only (never shipped, never a fallback: the product path is the HIP library).
  *
- * It runs floodca_amd/csrc/fca_walk.h's walk_strip with 64 lanes in lockstep
+ * It runs floodca_amd/csrc/fca_walk.h's walk_strip with 64 or 128 lanes in lockstep
  * (pull_left/pull_right are array shifts with 0 at the wavefront edge, as the
  * DPP shifts read 0), tile by tile in the order of the kernel's workgroups,
  * with the kernel's controller (ctrl_eval) and candidate rules. The CPU
@@ -17,34 +17,36 @@
 
 typedef real R;
 
+template <int W>
 struct CpuLanes {
-    WalkLane<R> s[FCA_WAVE];
+    static constexpr int WIDTH = W;
+    WalkLane<R> s[W];
     template <class F>
     void each(F f)
     {
-        for (int l = 0; l < FCA_WAVE; l++) {
+        for (int l = 0; l < W; l++) {
             f(s[l], l);
         }
     }
     template <class T>
     void pull_right(T WalkLane<R>::*dst, T WalkLane<R>::*src)
     {
-        T tmp[FCA_WAVE];
-        for (int l = 0; l < FCA_WAVE; l++) {
-            tmp[l] = (l + 1 < FCA_WAVE) ? s[l + 1].*src : (T)0;
+        T tmp[W];
+        for (int l = 0; l < W; l++) {
+            tmp[l] = (l + 1 < W) ? s[l + 1].*src : (T)0;
         }
-        for (int l = 0; l < FCA_WAVE; l++) {
+        for (int l = 0; l < W; l++) {
             s[l].*dst = tmp[l];
         }
     }
     template <class T>
     void pull_left(T WalkLane<R>::*dst, T WalkLane<R>::*src)
     {
-        T tmp[FCA_WAVE];
-        for (int l = 0; l < FCA_WAVE; l++) {
+        T tmp[W];
+        for (int l = 0; l < W; l++) {
             tmp[l] = (l > 0) ? s[l - 1].*src : (T)0;
         }
-        for (int l = 0; l < FCA_WAVE; l++) {
+        for (int l = 0; l < W; l++) {
             s[l].*dst = tmp[l];
         }
     }
@@ -54,25 +56,26 @@ struct CpuLanes {
         form_rows[form]++;
     }
     void loads_settled() {}
+    /* nonzero exactly when some lane qualifies (lanes folded onto 64 bits) */
     uint64_t land_lanes(int WalkLane<R>::*m)
     {
         uint64_t b = 0;
-        for (int l = 0; l < FCA_WAVE; l++) {
-            b |= (uint64_t)(s[l].*m == FCA_LAND) << l;
+        for (int l = 0; l < W; l++) {
+            b |= (uint64_t)(s[l].*m == FCA_LAND) << (l & 63);
         }
         return b;
     }
     uint64_t non_land_lanes(int WalkLane<R>::*m)
     {
         uint64_t b = 0;
-        for (int l = 0; l < FCA_WAVE; l++) {
-            b |= (uint64_t)(s[l].*m != FCA_LAND) << l;
+        for (int l = 0; l < W; l++) {
+            b |= (uint64_t)(s[l].*m != FCA_LAND) << (l & 63);
         }
         return b;
     }
     bool any(int WalkLane<R>::*f)
     {
-        for (int l = 0; l < FCA_WAVE; l++) {
+        for (int l = 0; l < W; l++) {
             if (s[l].*f) {
                 return true;
             }
@@ -82,10 +85,10 @@ struct CpuLanes {
     void wave_max_to(R WalkLane<R>::*dst, R WalkLane<R>::*src)
     {
         R v = s[0].*src;
-        for (int l = 1; l < FCA_WAVE; l++) {
+        for (int l = 1; l < W; l++) {
             v = (s[l].*src > v) ? s[l].*src : v;
         }
-        for (int l = 0; l < FCA_WAVE; l++) {
+        for (int l = 0; l < W; l++) {
             s[l].*dst = v;
         }
     }
@@ -106,6 +109,14 @@ extern "C" void emu_set_skip(int v)
 {
     emu_skip = v;
 }
+/* Lanes of the emulated wavefront: 64 (one column per work-item) or 128
+ * (two columns per work-item); tiles write width - 6 columns. */
+static int emu_width = 64;
+extern "C" void emu_set_width(int w)
+{
+    emu_width = (w == 128) ? 128 : 64;
+}
+
 /* Tiles skipped by the last emu_run. */
 extern "C" long emu_skipped_tiles(void)
 {
@@ -114,15 +125,15 @@ extern "C" long emu_skipped_tiles(void)
 
 /* The kernel's rain-free tile map (k_tile_norain). */
 static void tile_norain(const R *cr, const uint8_t *mask, int nr, int nc, int strip_rows,
-                        int ngroups, int ntiles, uint8_t *out)
+                        int ngroups, int ntiles, int outc, uint8_t *out)
 {
     for (int t = 0; t < ntiles; t++) {
         const int strip = t / ngroups, grp = t % ngroups;
         const int r0 = strip * strip_rows, r1 = (r0 + strip_rows < nr) ? r0 + strip_rows : nr;
         int rain = 0, land = 0;
         for (int r = r0; r < r1; r++) {
-            for (int l = 0; l < FCA_OUTC; l++) {
-                const int c = grp * FCA_OUTC + l;
+            for (int l = 0; l < outc; l++) {
+                const int c = grp * outc + l;
                 if (c < nc) {
                     const size_t i = (size_t)r * nc + c;
                     land |= mask[i] == FCA_LAND;
@@ -169,11 +180,14 @@ static int load_day(const char *path, R *dst, size_t n, R lambda)
     return 0;
 }
 
-/* Runs the fused step like fca_hip_run, on the host. g is updated in place
- * (h, hmax, tmax, Rday on reloads); acc as fca_hip_run sets it. */
-extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_rows,
-                       FcaHipStepRec *rec, int rec_cap, int *nrec)
+/* Runs the fused step like fca_hip_run, on the host, with wavefronts of W
+ * lanes. g is updated in place (h, hmax, tmax, Rday on reloads); acc as
+ * fca_hip_run sets it. */
+template <int W>
+static int emu_run_w(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_rows,
+                     FcaHipStepRec *rec, int rec_cap, int *nrec)
 {
+    const int outc = FCA_TILE_OUT(W);
     const int nr = g->nrows, nc = g->ncols, K8 = cfg->phys.K == 8;
     const size_t N = (size_t)nr * (size_t)nc;
     PhysK<R> p = make_phys<R>(cfg->phys);
@@ -223,7 +237,7 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
     /* the walk's LAND form depends on the data only; host arithmetic needs
      * none of the device's division flags */
     p.fast = emu_land_form == 0 ? 0 : fca_land_form_ok(&cfg->phys, g->n, g->mask, N);
-    ngroups = (nc + FCA_OUTC - 1) / FCA_OUTC;
+    ngroups = (nc + outc - 1) / outc;
     nstrips = (nr + strip_rows - 1) / strip_rows;
     uint8_t *tflags = (uint8_t *)calloc((size_t)2 * ngroups * nstrips, 1);
     uint8_t *tnorain = (uint8_t *)calloc((size_t)ngroups * nstrips, 1);
@@ -231,7 +245,7 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
         free(tflags), free(tnorain), free(h[0]), free(h[1]), free(cr);
         return -1;
     }
-    tile_norain(cr, g->mask, nr, nc, strip_rows, ngroups, ngroups * nstrips, tnorain);
+    tile_norain(cr, g->mask, nr, nc, strip_rows, ngroups, ngroups * nstrips, outc, tnorain);
     emu_skipped = 0;
 
     for (;;) {
@@ -249,7 +263,7 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
             for (i = 0; i < N; i++) {
                 cr[i] = g->C[i] * g->Rday[i];
             }
-            tile_norain(cr, g->mask, nr, nc, strip_rows, ngroups, ngroups * nstrips, tnorain);
+            tile_norain(cr, g->mask, nr, nc, strip_rows, ngroups, ngroups * nstrips, outc, tnorain);
             c.force_day = c.want_day;
             c.state = FCA_CTL_RUN;
             continue;
@@ -280,7 +294,7 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
             wa.nr = nr, wa.nc = nc, wa.strip_rows = strip_rows, wa.p = p, wa.sv = sv;
             wa.lgg = speed_log_half_g<R>(p.g);
             for (int t = 0; t < ngroups * nstrips; t++) {
-                static CpuLanes lp;
+                static CpuLanes<W> lp;
                 if (t == 0 && c.step == acc0.step) {
                     lp.form_rows[0] = lp.form_rows[1] = lp.form_rows[2] = 0;
                 }
@@ -298,20 +312,20 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
                     continue;
                 }
                 if (K8 && emu_skip) {
-                    walk_strip<R, 1, 1>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                    walk_strip<R, 1, 1>(lp, wa, r0, r1, grp * outc - FCA_HALO);
                 } else if (K8) {
-                    walk_strip<R, 1, 0>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                    walk_strip<R, 1, 0>(lp, wa, r0, r1, grp * outc - FCA_HALO);
                 } else if (emu_skip) {
-                    walk_strip<R, 0, 1>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                    walk_strip<R, 0, 1>(lp, wa, r0, r1, grp * outc - FCA_HALO);
                 } else {
-                    walk_strip<R, 0, 0>(lp, wa, r0, r1, grp * FCA_OUTC - FCA_HALO);
+                    walk_strip<R, 0, 0>(lp, wa, r0, r1, grp * outc - FCA_HALO);
                 }
                 cur[t] = (uint8_t)((lp.any(&WalkLane<R>::qwet) ? 0 : FCA_TILE_DRY) |
                                    (lp.any(&WalkLane<R>::qchg) ? 0 : FCA_TILE_SAME));
                 walk_acc_finish<R>(lp, p);
                 emu_rows[0] = lp.form_rows[0], emu_rows[1] = lp.form_rows[1];
                 emu_rows[2] = lp.form_rows[2];
-                for (int l = 0; l < FCA_WAVE; l++) {
+                for (int l = 0; l < W; l++) {
                     acc->V_rain += lp.s[l].v_rain;
                     acc->clamp_err += lp.s[l].clamp;
                     acc->V_sea += lp.s[l].v_sea;
@@ -357,4 +371,11 @@ extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_
     }
     free(h[0]), free(h[1]), free(cr), free(tflags), free(tnorain);
     return 0;
+}
+
+extern "C" int emu_run(FcaGrid *g, const FcaRunCfg *cfg, FcaAcc *acc, int strip_rows,
+                       FcaHipStepRec *rec, int rec_cap, int *nrec)
+{
+    return emu_width == 128 ? emu_run_w<128>(g, cfg, acc, strip_rows, rec, rec_cap, nrec)
+                            : emu_run_w<64>(g, cfg, acc, strip_rows, rec, rec_cap, nrec);
 }

@@ -159,3 +159,43 @@ extern "C" int mc_eval(int kind, uint32_t first, uint32_t count, const McArgs *a
     (void)hipFree(dv);
     return (int)e;
 }
+
+/* Every work-item of a full wavefront writes the wavefront's maximum of in
+ * (wave_max_nonneg, the stage-D threshold reduction). */
+template <class T>
+__global__ void k_wave_max(const T *in, T *out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    out[i] = wave_max_nonneg(in[i]);
+}
+
+/* n values (a multiple of 256, every wavefront full) of float (dbl = 0) or
+ * double (dbl = 1) from host in to host out. Returns 0 or a HIP error. */
+extern "C" int mc_wave_max(int dbl, const void *in, void *out, uint32_t n)
+{
+    if (n == 0 || n % 256 != 0) {
+        return (int)hipErrorInvalidValue;
+    }
+    const size_t bytes = (size_t)n * (dbl ? sizeof(double) : sizeof(float));
+    void *dv = nullptr;
+    hipError_t e = hipMalloc(&dv, 2 * bytes);
+    if (e != hipSuccess) {
+        return (int)e;
+    }
+    e = hipMemcpy(dv, in, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        if (dbl) {
+            hipLaunchKernelGGL(k_wave_max<double>, dim3(n / 256), dim3(256), 0, 0,
+                               (const double *)dv, (double *)dv + n);
+        } else {
+            hipLaunchKernelGGL(k_wave_max<float>, dim3(n / 256), dim3(256), 0, 0,
+                               (const float *)dv, (float *)dv + n);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        e = hipMemcpy(out, (char *)dv + bytes, bytes, hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(dv);
+    return (int)e;
+}

@@ -1,8 +1,8 @@
 """The fused kernel's dataflow (floodca_amd/csrc/fca_walk.h) on 64 emulated
-lanes against the oracle, bit for bit (CPU): strips and column groups cut
-anywhere, degenerate grids, both neighbourhoods and boundaries, both
-precisions. The GPU runs the same template with DPP shifts in place of the
-emulated ones."""
+lanes (one column per work-item) and on 128 (two), against the oracle, bit
+for bit (CPU): strips and column groups cut anywhere, degenerate grids, both
+neighbourhoods and boundaries, both precisions. The GPU runs the same
+template with DPP shifts in place of the emulated ones."""
 import numpy as np
 import pytest
 
@@ -13,18 +13,19 @@ from oracle import Oracle
 SHAPES = [(61, 130, 16), (97, 121, 5), (7, 5, 3), (1, 9, 4), (40, 1, 7), (2, 2, 1), (3, 64, 2)]
 
 
+@pytest.mark.parametrize("width", [64, 128])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("K", [4, 8])
 @pytest.mark.parametrize("bnd", [A.FCA_BND_OPEN, A.FCA_BND_CLOSED])
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}s{s[2]}")
-def test_emulated_walk_matches_oracle(prec, K, bnd, shape):
+def test_emulated_walk_matches_oracle(prec, K, bnd, shape, width):
     nr, nc, strip = shape
     a, g, cfg = make_case(nr, nc, prec=prec, K=K, boundary=bnd, T_sim=2400.0, walls=nr > 10,
                           seed=nr * 7 + nc)
     o = Oracle(prec)
     go, ge = g.copy(), g.copy()
     acc_o, log_o = o.run(go, cfg, log_cap=100000)
-    acc_e, log_e = emu_run(ge, cfg, o.acc_init(ge, cfg), prec, strip_rows=strip)
+    acc_e, log_e = emu_run(ge, cfg, o.acc_init(ge, cfg), prec, strip_rows=strip, width=width)
     for k in ("h", "hmax", "tmax"):
         assert np.array_equal(getattr(go, k), getattr(ge, k)), k
     assert [r.dt for r in log_o] == [r.dt for r in log_e]
@@ -32,14 +33,18 @@ def test_emulated_walk_matches_oracle(prec, K, bnd, shape):
     assert acc_o.V_rain == pytest.approx(acc_e.V_rain, rel=1e-12)
 
 
-LAND_SHAPES = [(70, 250, 32), (45, 180, 9)]
+# (wavefront width, shape): each wide enough for an inner column group of
+# that width (the LAND form's condition)
+LAND_SHAPES = [(64, (70, 250, 32)), (64, (45, 180, 9)), (128, (70, 250, 32)),
+               (128, (45, 300, 9))]
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("K", [4, 8])
 @pytest.mark.parametrize("bnd", [A.FCA_BND_OPEN, A.FCA_BND_CLOSED])
-@pytest.mark.parametrize("shape", LAND_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}s{s[2]}")
-def test_land_form_matches_oracle(prec, K, bnd, shape):
+@pytest.mark.parametrize("width,shape", LAND_SHAPES,
+                         ids=lambda s: f"{s[0]}x{s[1]}s{s[2]}" if isinstance(s, tuple) else s)
+def test_land_form_matches_oracle(prec, K, bnd, shape, width):
     """Mostly-LAND grids wide enough for inner column groups, so that rows take
     the walk's LAND form next to general rows (coast, walls, edges); the
     result is the oracle's bit for bit, and the same with the LAND form off."""
@@ -53,10 +58,11 @@ def test_land_form_matches_oracle(prec, K, bnd, shape):
     go, ge, gg = g.copy(), g.copy(), g.copy()
     acc_o, log_o = o.run(go, cfg, log_cap=100000)
     forms = []
-    acc_e, log_e = emu_run(ge, cfg, o.acc_init(ge, cfg), prec, strip_rows=strip, forms=forms)
+    acc_e, log_e = emu_run(ge, cfg, o.acc_init(ge, cfg), prec, strip_rows=strip, forms=forms,
+                           width=width)
     assert forms[1] > 0 and forms[0] > 0, forms
     acc_g, log_g = emu_run(gg, cfg, o.acc_init(gg, cfg), prec, strip_rows=strip,
-                           land_form=False)
+                           land_form=False, width=width)
     for k in ("h", "hmax", "tmax"):
         assert np.array_equal(getattr(go, k), getattr(ge, k)), k
         assert np.array_equal(getattr(gg, k), getattr(ge, k)), k

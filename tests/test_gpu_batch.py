@@ -32,15 +32,17 @@ def scenarios(prec, K):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-@pytest.mark.parametrize("K", [4, 8])
-def test_batch_equals_each_scenario_alone(prec, K):
+@pytest.mark.parametrize("prec,K,cpl", [("f32", 4, 1), ("f32", 8, 1), ("f64", 4, 1),
+                                        ("f64", 8, 1), ("f32", 4, 2), ("f32", 8, 2)])
+def test_batch_equals_each_scenario_alone(prec, K, cpl):
+    """cpl: columns per work-item of the walk, in the batch and alone."""
     o = Oracle(prec)
     cases = scenarios(prec, K)
     alone = []
     for g, cfg in cases:
         ga = g.copy()
-        with Sim(ga, cfg, o.acc_init(ga, cfg), prec=prec, strip_rows=16, record_steps=CAP) as s:
+        with Sim(ga, cfg, o.acc_init(ga, cfg), prec=prec, strip_rows=16, record_steps=CAP,
+                 cols_per_lane=cpl) as s:
             s.run()
             acc = s.read()
             log = s.steplog(CAP)
@@ -48,7 +50,7 @@ def test_batch_equals_each_scenario_alone(prec, K):
     grids = [g.copy() for g, _ in cases]
     accs = [o.acc_init(g, cfg) for g, (_, cfg) in zip(grids, cases)]
     with Batch(grids, [cfg for _, cfg in cases], accs, prec=prec, strip_rows=16,
-               record_steps=CAP) as b:
+               record_steps=CAP, cols_per_lane=cpl) as b:
         b.run()
         for i, (ga, acc_a, log_a) in enumerate(alone):
             acc_b = b.read(i)

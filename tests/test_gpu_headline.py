@@ -4,8 +4,9 @@ would take minutes a step, checked through properties that do not depend on
 size:
 
 * the same input gives the same bits across tile shapes: the fused walk with
-  strips of 96 and of 64 rows ends with identical h, hmax, tmax and time-step
-  sequences (no tile reads what another writes in the same step);
+  strips of 96 and of 64 rows, and with two columns per work-item (tiles of
+  122 columns), ends with identical h, hmax, tmax and time-step sequences
+  (no tile reads what another writes in the same step);
 * the split kernels, the independent second implementation (face arrays, one
   launch per phase), end with the same bits too;
 * every depth stays >= 0 and the mass-balance identity (diag.c:5-14) holds
@@ -57,23 +58,27 @@ def test_headline_grid_same_bits_across_tiles_and_kernels(grid, K):
     o96, acc96, log96 = _run(a, g, K, strip_rows=96)
     print(f"case headline {SIZE}^2 K={K}: fused strips of 64", flush=True)
     o64, acc64, log64 = _run(a, g, K, strip_rows=64)
+    print(f"case headline {SIZE}^2 K={K}: fused, two columns per work-item", flush=True)
+    o2, acc2, log2 = _run(a, g, K, cols_per_lane=2)
     print(f"case headline {SIZE}^2 K={K}: split kernels", flush=True)
     osp, accsp, logsp = _run(a, g, K, kernel=A.FCA_HIP_SPLIT)
-    assert acc96.step == acc64.step == accsp.step == STEPS
-    assert log96 == log64 == logsp
+    assert acc96.step == acc64.step == acc2.step == accsp.step == STEPS
+    assert log96 == log64 == log2 == logsp
     for k in ("h", "hmax", "tmax"):
         assert np.array_equal(getattr(o96, k), getattr(o64, k)), k
+        assert np.array_equal(getattr(o96, k), getattr(o2, k)), k
         assert np.array_equal(getattr(o96, k), getattr(osp, k)), k
     land = g.mask == A.FCA_LAND
     assert o96.h.min() >= 0.0
     assert np.all(o96.h[~land] == g.h[~land])   # SEA and WALL depths never change
     assert np.count_nonzero(o96.h[land] >= 1e-3) > 0.99 * land.sum()
-    for acc in (acc96, acc64, accsp):
+    for acc in (acc96, acc64, acc2, accsp):
         e = acc.V + acc.V_sea + acc.V_bnd - acc.V_rain - acc.V0 - acc.clamp_err
         assert abs(e) <= 1e-6 * (acc.V_rain + acc.V0)
         assert acc.cap_active == acc96.cap_active and acc.cap_total == acc96.cap_total
     for k in ("V", "V_rain", "V_sea", "V_bnd", "clamp_err"):
         assert getattr(acc64, k) == pytest.approx(getattr(acc96, k), rel=1e-12, abs=1e-6), k
+        assert getattr(acc2, k) == pytest.approx(getattr(acc96, k), rel=1e-12, abs=1e-6), k
         assert getattr(accsp, k) == pytest.approx(getattr(acc96, k), rel=1e-9, abs=1e-6), k
     # the faces carried water: under uniform rain alone every LAND depth
     # would be the same

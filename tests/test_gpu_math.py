@@ -352,3 +352,37 @@ def test_certified_cube_root_on_every_depth():
         deferred += int(np.count_nonzero(~ok))
         total += n
     assert deferred < total * 2.0 ** -10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_wave_max_is_the_wavefront_maximum(dtype):
+    """The stage-D threshold reduction: every lane of each 64-lane wavefront
+    gets the largest of the wavefront's non-negative values, wherever that
+    maximum sits (one wavefront per lane position), among zeros, subnormals,
+    ties and random values."""
+    lib = _lib()
+    lib.mc_wave_max.restype = C.c_int
+    lib.mc_wave_max.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_uint32]
+    rng = np.random.default_rng(7)
+    waves = []
+    for pos in range(64):  # the maximum at each lane in turn
+        w = rng.random(64).astype(dtype)
+        w[pos] = dtype(2.0)
+        waves.append(w)
+    waves.append(np.zeros(64, dtype))
+    w = np.zeros(64, dtype)
+    w[63] = np.finfo(dtype).smallest_subnormal
+    waves.append(w)
+    w = np.zeros(64, dtype)
+    w[0] = np.finfo(dtype).smallest_subnormal
+    waves.append(w)
+    waves.append(np.full(64, 3.5, dtype))
+    while len(waves) % 4:
+        waves.append((rng.random(64) * 10.0 ** rng.integers(-30, 30, 64)).astype(dtype))
+    x = np.ascontiguousarray(np.stack(waves))
+    out = np.empty_like(x)
+    rc = lib.mc_wave_max(int(dtype == np.float64), x.ctypes.data, out.ctypes.data, x.size)
+    assert rc == 0, f"mc_wave_max failed: HIP error {rc}"
+    want = np.repeat(x.max(axis=1, keepdims=True), 64, axis=1)
+    assert np.array_equal(out.view(np.uint8), want.view(np.uint8))

@@ -38,13 +38,13 @@ def _ids(c):
            f"{'fused' if kern == 0 else 'split'}-{prec}"
 
 
-def run_both(a, g, cfg, prec, kernel=A.FCA_HIP_FUSED, strip=0, cap=100000):
+def run_both(a, g, cfg, prec, kernel=A.FCA_HIP_FUSED, strip=0, cap=100000, cols_per_lane=0):
     o = Oracle(prec)
     go, gh = g.copy(), g.copy()
     acc_o, log_o = o.run(go, cfg, log_cap=cap)
     acc_h = o.acc_init(gh, cfg)
     with Sim(gh, cfg, acc_h, prec=prec, kernel=kernel, strip_rows=strip,
-             record_steps=cap) as s:
+             record_steps=cap, cols_per_lane=cols_per_lane) as s:
         s.run()
         acc_g = s.read()
         log_h = s.steplog(cap)
@@ -91,3 +91,26 @@ def test_fixed_steps_triangle_events(prec):
                           phi=0.25, out_every=900.0, T_sim=1e9, steps=400, U=3, rday=40.0)
     print(f"case triangle/U3/{prec}", flush=True)
     assert_parity(prec, *run_both(a, g, cfg, prec, A.FCA_HIP_FUSED, 16))
+
+
+# the walk with two columns per work-item (tiles of 122 columns): grids with
+# inner column groups of that width (LAND form), narrow and degenerate ones
+CPL2_CASES = [
+    # nr, nc, K, boundary, strip
+    (61, 47, 4, A.FCA_BND_OPEN, 0),
+    (61, 300, 4, A.FCA_BND_CLOSED, 16),
+    (97, 400, 8, A.FCA_BND_OPEN, 8),
+    (45, 260, 8, A.FCA_BND_CLOSED, 0),
+    (7, 5, 4, A.FCA_BND_OPEN, 3),
+    (40, 1, 8, A.FCA_BND_OPEN, 7),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("c", CPL2_CASES, ids=lambda c: f"{c[0]}x{c[1]}-K{c[2]}-b{c[3]}-s{c[4]}")
+def test_step_parity_two_columns_per_lane(c):
+    nr, nc, K, b, strip = c
+    a, g, cfg = make_case(nr, nc, prec="f32", K=K, boundary=b, T_sim=3600.0, land_frac=0.9,
+                          h0=0.05)
+    print(f"case cpl2 {nr}x{nc} K={K} b={b} strip={strip}", flush=True)
+    assert_parity("f32", *run_both(a, g, cfg, "f32", A.FCA_HIP_FUSED, strip, cols_per_lane=2))

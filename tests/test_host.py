@@ -93,6 +93,18 @@ def test_library_rejects_bad_configuration_without_a_device(bad, msg):
         Sim(g, cfg, acc)
 
 
+@pytest.mark.parametrize("prec,cpl,msg", [
+    ("f32", 3, "cols_per_lane must be 0, 1 or 2"),
+    ("f32", -1, "cols_per_lane must be 0, 1 or 2"),
+    ("f64", 2, "cols_per_lane 2 is built for the float library only"),
+])
+def test_library_rejects_bad_columns_per_lane_without_a_device(prec, cpl, msg):
+    a, g, cfg = make_case(8, 8, prec=prec)
+    acc = a.FcaAcc()
+    with pytest.raises(HipError, match=msg):
+        Sim(g, cfg, acc, prec=prec, cols_per_lane=cpl)
+
+
 def test_library_rejects_empty_grid():
     a = A.abi("f32")
     b = backend("f32")
