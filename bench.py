@@ -135,15 +135,20 @@ VALU_ISSUE_PER_S = 256 * 2.4e9  # wave64 VALU instructions per second: one per C
 
 
 def load_valu(workload: str):
-    """wave64 VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU) for
-    this workload, as committed under profiles/ (None if not measured)."""
+    """(wave64 VALU instructions per launch, shader clock in Hz during the
+    launch) for this workload, as committed under profiles/ (rocprofv3 --pmc
+    SQ_INSTS_VALU; GRBM_GUI_ACTIVE per XCD over the dispatch's duration);
+    None for what was not measured."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p, encoding="utf-8") as fp:
             t = json.load(fp).get(workload)
     except (OSError, ValueError):
-        return None
-    return None if t is None or "valu_insts" not in t else float(t["valu_insts"])
+        return None, None
+    if t is None:
+        return None, None
+    return (float(t["valu_insts"]) if "valu_insts" in t else None,
+            float(t["clock_hz"]) if "clock_hz" in t else None)
 
 
 def cpu_baseline(seconds: float, prec: str):
@@ -642,7 +647,7 @@ def main() -> int:
     default_kernel = (kern == A.FCA_HIP_FUSED and args.strip == 0 and not args.no_land_form
                       and args.cols_per_lane in (0, 1))
     traffic = load_traffic(workload) if default_kernel else None
-    valu = load_valu(workload) if default_kernel else None
+    valu, clock_hz = load_valu(workload) if default_kernel else (None, None)
     emass = abs(out.V + out.V_sea + out.V_bnd - out.V_rain - out.V0 - out.clamp_err) / \
         max(out.V_rain + out.V0, 1e-30)
     print(f"[bench] {ms_per_step:.4f} ms/step, {achieved:.0f} GB/s algorithmic, "
@@ -684,7 +689,12 @@ def main() -> int:
             "issue": (None if valu is None else {
                 "valu_insts_per_launch": valu, "issue_ms": valu / VALU_ISSUE_PER_S * 1e3,
                 "frac": valu / VALU_ISSUE_PER_S / (ms_per_step * 1e-3),
-                "rate_per_s": VALU_ISSUE_PER_S}),
+                "rate_per_s": VALU_ISSUE_PER_S,
+                # the same ceiling at the clock the counters measured during
+                # the launch (256 CUs x clock)
+                "clock_hz": clock_hz,
+                "frac_at_clock": (None if clock_hz is None else
+                                  valu / (256 * clock_hz) / (ms_per_step * 1e-3))}),
         },
     }
     if args.grid4096 and args.size != 4096 and prec == "f32" and args.neigh == 4:
