@@ -253,13 +253,52 @@ __device__ inline R wave_max_nonneg(R v)
     v = fmax_r(v, dpp_or_zero<0x143, 0xc>(v)); /* row_bcast:31 */
     return lane63(v);
 }
+/* The float form in integers: floats >= +0 are ordered as their bit patterns
+ * are as signed integers, so an integer maximum gives the float maximum with
+ * no NaN quieting of its operands (which a float maximum of shifted or
+ * loop-carried values needs), and +0 is the integer 0. */
+__device__ inline float wave_max_nonneg(float v)
+{
+    int b = __float_as_int(v);
+    b = max(b, __builtin_amdgcn_update_dpp(0, b, 0x111, 0xf, 0xf, true)); /* row_shr:1 */
+    b = max(b, __builtin_amdgcn_update_dpp(0, b, 0x112, 0xf, 0xf, true)); /* row_shr:2 */
+    b = max(b, __builtin_amdgcn_update_dpp(0, b, 0x114, 0xf, 0xf, true)); /* row_shr:4 */
+    b = max(b, __builtin_amdgcn_update_dpp(0, b, 0x118, 0xf, 0xf, true)); /* row_shr:8 */
+    b = max(b, __builtin_amdgcn_update_dpp(0, b, 0x142, 0xa, 0xf, true)); /* row_bcast:15 */
+    b = max(b, __builtin_amdgcn_update_dpp(0, b, 0x143, 0xc, 0xf, true)); /* row_bcast:31 */
+    return __int_as_float(__builtin_amdgcn_readlane(b, 63));
+}
 #endif
+
+/* A wavefront-wide test inside a lane's code. On the device: uniform, whether
+ * c holds on some active lane. On the host the emulation runs the lanes one
+ * at a time, so it is c itself, and a branch on wave_any followed by a test
+ * of c takes the same lanes either way. */
+FCA_HDI bool wave_any(bool c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(c) != 0;
+#else
+    return c;
+#endif
+}
 
 /* x if x > 0, else +0: the term an outflow sum adds for x. */
 template <class R>
 FCA_HDI R pos_part(R x)
 {
     return (x > (R)0) ? x : (R)0;
+}
+/* float: the same value by an integer maximum of the bit pattern (a positive
+ * float is a positive integer; -0 and negative floats are negative integers,
+ * whose maximum with 0 is +0); no NaN quieting of x is needed. */
+FCA_HDI float pos_part(float x)
+{
+    int32_t b;
+    __builtin_memcpy(&b, &x, 4);
+    b = (b > 0) ? b : 0;
+    __builtin_memcpy(&x, &b, 4);
+    return x;
 }
 
 /* Rough sqrt (relative error < 2^-21) used only to skip exact work. */
@@ -334,13 +373,16 @@ FCA_HD float cbrt_sq_fix(float y, double x)
  * of a float rounding midpoint. */
 #define FCA_CBRT_GUARD 16384
 
-/* One Newton step from the hardware seed: a^(2/3) to ~2^-41 relative. */
+/* One Newton step from the hardware seed: a^(2/3) to ~2^-41 relative. The
+ * step y0 + r/(3 y0^2) is one fma (one rounding fewer than a product and a
+ * sum, so within the same bound), and r = a^2 - y0^3 another (y0^2 is exact
+ * in double). */
 __device__ inline double cbrt_sq_newton(float a, double sq)
 {
     const float y0 = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(a) * (2.0f / 3.0f));
     const double yd = (double)y0;
     const double r = __builtin_fma(-(yd * yd), yd, sq); /* a^2 - y0^3, one rounding */
-    return yd + r * (double)__builtin_amdgcn_rcpf(3.0f * y0 * y0);
+    return __builtin_fma(r, (double)__builtin_amdgcn_rcpf(3.0f * y0 * y0), yd);
 }
 #endif
 
@@ -663,13 +705,16 @@ FCA_HDI R face_land_exact(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R
  * face_land_exact's, bit for bit. */
 template <class R>
 FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, R rden,
-                    int den_fast, const PhysK<R> &p, int &cap)
+                    int den_fast, const PhysK<R> &p, bool &cap)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     if (sizeof(R) == 4) {
         const float dL = la - lb;
+        /* la > lb exactly when la - lb > 0 (finite floats, gradual
+         * underflow), so one compare serves the upper level and the sign */
+        const bool up = dL > 0.0f;
         const float adL = __builtin_fabsf(dL);
-        const float hf = ((la > lb) ? la : lb) - ((za > zb) ? za : zb);
+        const float hf = (up ? la : lb) - ((za > zb) ? za : zb);
         /* flags are bools combined with && and ||, so each stays a lane mask
          * and its ballot costs no vector instruction */
         const bool live = !(adL < (float)p.tau_l) && !(hf < (float)p.tau_h);
@@ -710,9 +755,9 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
             mfix |= lane_mask(!(x >= 0x1p-80f)) | lane_mask(!(x <= 0x1p80f));
             c0 = x * (float)rden;
             qcap = __builtin_fmaf(__builtin_fmaf(-c0, (float)den, x), (float)rden, c0);
-            cap = (live && qcap < qmag) ? 1 : 0;
+            cap = live && qcap < qmag;
             qmag = (qcap < qmag) ? qcap : qmag;
-            qmag = (dL > 0.0f) ? qmag : -qmag;
+            qmag = up ? qmag : -qmag;
             if ((mfix & mlive) != 0) {
                 FCA_COLD_PATH();
                 const uint64_t lane_bit = 1ull << __builtin_amdgcn_mbcnt_hi(
@@ -721,7 +766,7 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
                     int cx;
                     const R q = face_land_exact<R>(za, la, na, zb, lb, nb, d, rd, dt, den, rden,
                                                    den_fast, p, cx);
-                    cap = cx;
+                    cap = cx != 0;
                     return q;
                 }
             }
@@ -729,7 +774,10 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
         }
     }
 #endif
-    return face_land_exact<R>(za, la, na, zb, lb, nb, d, rd, dt, den, rden, den_fast, p, cap);
+    int cx;
+    const R q = face_land_exact<R>(za, la, na, zb, lb, nb, d, rd, dt, den, rden, den_fast, p, cx);
+    cap = cx != 0;
+    return q;
 }
 
 /* Open-boundary virtual face of a LAND edge cell: z_b = z_a - S_bnd*dx,

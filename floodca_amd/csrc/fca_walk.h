@@ -333,20 +333,21 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
             lp.pull_right(&L::xs, &L::n1);
         }
         lp.each([&](L &s, int) {
-            int cap, ct = 0;
+            int ct = 0;
+            bool cb;
             s.qx0 = face_land<R>(s.z0, s.lv0, s.n0, s.xz, s.xl, s.xn, p.dx, p.rdx, dt, den, rden,
-                                 dfast, p, cap);
-            ct += outA ? cap : 0;
+                                 dfast, p, cb);
+            ct += (outA && cb) ? 1 : 0;
             s.qy1 = face_land<R>(s.z1, s.lv1, s.n1, s.z0, s.lv0, s.n0, p.dx, p.rdx, dt, den,
-                                 rden, dfast, p, cap);
-            ct += outB ? cap : 0;
+                                 rden, dfast, p, cb);
+            ct += (outB && cb) ? 1 : 0;
             if (K8) {
                 s.qd1_1 = face_land<R>(s.z1, s.lv1, s.n1, s.xz, s.xl, s.xn, p.ddiag, p.rddiag,
-                                       dt, den, rden, dfast, p, cap);
-                ct += outB ? cap : 0;
+                                       dt, den, rden, dfast, p, cb);
+                ct += (outB && cb) ? 1 : 0;
                 s.qd2_1 = face_land<R>(s.xq, s.xl1, s.xs, s.z0, s.lv0, s.n0, p.ddiag, p.rddiag,
-                                       dt, den, rden, dfast, p, cap);
-                ct += outB ? cap : 0;
+                                       dt, den, rden, dfast, p, cb);
+                ct += (outB && cb) ? 1 : 0;
             }
             if (s.outcol) {
                 s.cap += ct;
@@ -580,21 +581,26 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 return;
             }
             {
-                double clamp_here = 0.0;
                 rain = s.pcr * sv.rain_f;
-                hnew = s.h2 + (dt * (LANDF ? div_c_wave(net, A, p.rA) : div_c(net, A, p.rA, p.fA)) +
-                               rain);
-                if (hnew < (R)0) {
-                    clamp_here = (double)(-hnew) * (double)A;
-                    hnew = (R)0;
-                }
+                const R hraw =
+                    s.h2 + (dt * (LANDF ? div_c_wave(net, A, p.rA) : div_c(net, A, p.rA, p.fA)) +
+                            rain);
+                const bool neg = hraw < (R)0; /* clamped to +0 */
+                hnew = neg ? (R)0 : hraw;
                 s.hn2 = hnew;
                 if (outC && s.outcol) {
                     /* rose in the previous step (h_old > stored hmax) and
                      * not in this one: store hmax = h_old and its tmax */
                     const int stop = sv.armed && s.h2 > s.phm && !(hnew > s.h2);
                     s.v_rain += (double)A * (double)rain;
-                    s.clamp += clamp_here;
+                    /* the clamped volume is added only where some lane of
+                     * the wavefront clamps (adding +0 elsewhere would leave
+                     * the non-negative sum as it is) */
+                    if (wave_any(neg)) {
+                        if (neg) {
+                            s.clamp += (double)(-hraw) * (double)A;
+                        }
+                    }
                     s.stq = stop ? 3 : 1;
                     if (QT) {
                         /* changed: h, a store, or the true hmax (h_new >
