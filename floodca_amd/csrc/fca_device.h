@@ -64,8 +64,14 @@ typedef struct FcaAccSlot {
 static inline int fca_land_phys_ok(const FcaPhys *ph)
 {
     const real dx = ph->dx, dmax = (ph->K == 8) ? (real)sqrt(2.0) * dx : dx;
+    const real A = ph->A;
+    /* the row test of the LAND form (|z + h| <= 2^20, |h| <= 2^14 on every
+     * lane, fca_walk.h) bounds a live face's |dL| by 2^21.1 and hf by
+     * 2^14 + 1; with A <= 2^58 and dx >= 2^-50 this keeps RN(|dL|*A) <= 2^80
+     * and |dL|/d <= 2^126, and RN(tau_l*A) >= 2^-80 keeps RN(|dL|*A) >= 2^-80 */
     return ph->tau_l >= (real)0x1p-80 && ph->tau_l / dmax >= (real)0x1p-96 &&
-           ph->tau_h >= (real)0x1p-15 && ph->g >= (real)0x1p-80 && ph->g <= (real)0x1p80;
+           ph->tau_h >= (real)0x1p-15 && ph->g >= (real)0x1p-80 && ph->g <= (real)0x1p80 &&
+           A <= (real)0x1p58 && dx >= (real)0x1p-50 && (real)(ph->tau_l * A) >= (real)0x1p-80;
 }
 static inline int fca_land_n_ok(const real *n, const uint8_t *mask, size_t N)
 {
@@ -115,6 +121,9 @@ struct PhysK {
     /* the LAND form may be used: fca_land_form_ok and, in float, fdx, fA
      * and (K=8) fddiag */
     int fast;
+    /* RN(g*(1-2^-16)): the LAND form's test that a face speed is clearly
+     * below the critical speed (face_land) */
+    R gk;
 };
 
 /* a/d for a run-constant d with reciprocal rd = RN(1/d). With ok set (the
@@ -728,20 +737,28 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
              * every value they produce, their flags included, is masked by
              * the live lanes below, and floating-point exceptions do not trap.
              * mfix collects, one compare at a time, the lanes whose certificate
-             * fails or whose input leaves the ranges the fast forms assume */
-            uint64_t mfix = lane_mask(!(adL <= 0x1p80f)) | (den_fast ? 0ull : ~0ull);
+             * fails. The ranges the fast forms assume hold on every live lane
+             * by the LAND form's row test (|z + h| <= 2^20 and |h| <= 2^14,
+             * so |dL| < 2^21.1 and hf <= 2^14 + 1) and the run constants
+             * fca_land_phys_ok admits: hf in [2^-15, 2^15] for the cube
+             * root, |dL|/d in [2^-96, 2^126] for the square root, RN(|dL|*A)
+             * in [2^-80, 2^80] for Qcap, g*hf in [2^-100, 2^100] for the
+             * critical speed */
+            uint64_t mfix = den_fast ? 0ull : ~0ull;
             const float nf = 0.5f * ((float)na + (float)nb);
             const float q0 = adL * (float)rd;
             const float sl = __builtin_fmaf(__builtin_fmaf(-q0, (float)d, adL), (float)rd, q0);
             const double y1 = cbrt_sq_newton(hf, (double)hf * (double)hf);
-            mfix |= lane_mask(near_float_midpoint(y1)) | lane_mask(!(hf <= FCA_CBRT_HI));
+            mfix |= lane_mask(near_float_midpoint(y1));
             float v = rcp_in_range(nf) * (float)y1 * sqrt_in_range(sl);
             float qmag, x, c0, qcap;
             {
                 /* the critical speed only where some lane's v is not clearly
-                 * below it (crit_cap's test): a uniform branch */
-                const float gh = (float)p.g * hf;
-                if ((lane_mask(!(v * v < gh * (1.0f - 0x1p-16f))) & mlive) != 0) {
+                 * below it, a uniform branch. crit_cap's test with the
+                 * constant folded: gk = RN(g*(1-2^-16)), and fl(v*v) <
+                 * fl(hf*gk) gives v^2 < g*hf*(1-2^-16)(1+2^-24)^3 <
+                 * g*hf*(1-2^-17), so v < sqrt(g*hf)*(1-2^-18) < vcr */
+                if ((lane_mask(!(v * v < hf * (float)p.gk)) & mlive) != 0) {
                     bool cfix = false;
                     const float vcr = crit_speed_cert((float)p.g, hf, cfix);
                     mfix |= lane_mask(cfix);
@@ -752,7 +769,6 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
             }
             qmag = v * hf * (float)p.w;
             x = adL * (float)p.A;
-            mfix |= lane_mask(!(x >= 0x1p-80f)) | lane_mask(!(x <= 0x1p80f));
             c0 = x * (float)rden;
             qcap = __builtin_fmaf(__builtin_fmaf(-c0, (float)den, x), (float)rden, c0);
             cap = live && qcap < qmag;
@@ -1177,6 +1193,7 @@ inline PhysK<R> make_phys(const FcaPhys &f)
     p.rdx = (R)1 / p.dx, p.rddiag = (R)1 / p.ddiag, p.rA = (R)1 / p.A;
     p.fdx = p.fddiag = p.fA = 0;
     p.fast = 0;
+    p.gk = p.g * ((R)1 - (R)0x1p-16);
     return p;
 }
 

@@ -69,11 +69,41 @@ __device__ __forceinline__ double lane_from_left(double x)
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+/* Lanes (of a wavefront) whose level z + h or depth h leaves the LAND
+ * form's range: |z + h| <= 2^20, |h| <= 2^14 (fca_device.h, face_land). Each
+ * compare is balloted on its own, so the masks cost no vector select. */
+__device__ __forceinline__ uint64_t lanes_out_of_range(float lv, float h)
+{
+    return __builtin_amdgcn_ballot_w64(!(__builtin_fabsf(lv) <= 0x1p20f)) |
+           __builtin_amdgcn_ballot_w64(!(__builtin_fabsf(h) <= 0x1p14f));
+}
+__device__ __forceinline__ uint64_t lanes_out_of_range(double, double) { return 0; }
+
 /* Lane policy of the walk on the GPU: one work-item is one lane. */
 struct GpuLanes {
     static constexpr int WIDTH = 64;
     WalkLane<R> s;
     int lane;
+    uint64_t outmask; /* the lanes whose column the strip writes */
+    __device__ __forceinline__ void strip_begun()
+    {
+        outmask = __builtin_amdgcn_ballot_w64(s.outcol != 0);
+    }
+    /* capped faces of written columns where c holds (all lanes active): a
+     * count per wavefront, kept in scalar registers, in capw */
+    __device__ __forceinline__ void count_caps(WalkLane<R> &t, bool c)
+    {
+        t.capw += (long long)__builtin_popcountll(__builtin_amdgcn_ballot_w64(c) & outmask);
+    }
+    /* the work-item's share of the capped-face count */
+    __device__ __forceinline__ long long cap_share() const
+    {
+        return s.cap + (lane == 0 ? s.capw : 0);
+    }
+    __device__ __forceinline__ uint64_t out_of_range_lanes()
+    {
+        return lanes_out_of_range(s.lv0, s.h0);
+    }
     template <class F>
     __device__ __forceinline__ void each(F f)
     {
@@ -150,6 +180,25 @@ struct GpuLanes2 {
     static constexpr int WIDTH = 128;
     WalkLane<R> s[2];
     int lane;
+    uint64_t outmask[2];
+    __device__ __forceinline__ void strip_begun()
+    {
+        outmask[0] = __builtin_amdgcn_ballot_w64(s[0].outcol != 0);
+        outmask[1] = __builtin_amdgcn_ballot_w64(s[1].outcol != 0);
+    }
+    __device__ __forceinline__ void count_caps(WalkLane<R> &t, bool c)
+    {
+        const uint64_t m = (&t == &s[0]) ? outmask[0] : outmask[1];
+        t.capw += (long long)__builtin_popcountll(__builtin_amdgcn_ballot_w64(c) & m);
+    }
+    __device__ __forceinline__ long long cap_share() const
+    {
+        return s[0].cap + s[1].cap + (lane == 0 ? s[0].capw + s[1].capw : 0);
+    }
+    __device__ __forceinline__ uint64_t out_of_range_lanes()
+    {
+        return lanes_out_of_range(s[0].lv0, s[0].h0) | lanes_out_of_range(s[1].lv0, s[1].h0);
+    }
     template <class F>
     __device__ __forceinline__ void each(F f)
     {
@@ -406,7 +455,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
         typedef WalkLane<R> L;
         double vr = wave_sum(lp.item_sum(&L::v_rain)), cl = wave_sum(lp.item_sum(&L::clamp));
         double vs = wave_sum(lp.item_sum(&L::v_sea)), vb = wave_sum(lp.item_sum(&L::v_bnd));
-        long long cp = wave_sum(lp.item_sum(&L::cap));
+        long long cp = wave_sum(lp.cap_share());
         R dm = wave_min(lp.item_min(&L::dtc));
         if (lane == 0) {
             sh_acc[wave][0] = vr, sh_acc[wave][1] = cl, sh_acc[wave][2] = vs;

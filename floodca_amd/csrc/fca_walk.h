@@ -110,9 +110,14 @@ struct WalkLane {
     /* this tile's output cells: some depth >= tau_h after the step (qwet),
      * some h, hmax or tmax changed by the step (qchg) */
     int qwet, qchg;
-    /* accumulators of this lane's output cells */
+    /* accumulators of this lane's output cells; v_rain and clamp hold sums
+     * of depths (rain, and -h of clamped cells), scaled by A once the walk
+     * ends (walk_acc_finish): each product (double)A*(double)x of two floats
+     * is exact, so the scaling changes only the sums' roundings. cap counts
+     * capped faces per lane, capw per wavefront where the lane policy counts
+     * them with ballots (count_caps) */
     double v_rain, clamp, v_sea, v_bnd;
-    long long cap;
+    long long cap, capw;
     R dtc, vmax, vthr;
     R lthr; /* log2 form of vthr for speed_may_exceed_fast (vthr and lthr are
              * the same on every lane of a wavefront) */
@@ -333,24 +338,20 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
             lp.pull_right(&L::xs, &L::n1);
         }
         lp.each([&](L &s, int) {
-            int ct = 0;
             bool cb;
             s.qx0 = face_land<R>(s.z0, s.lv0, s.n0, s.xz, s.xl, s.xn, p.dx, p.rdx, dt, den, rden,
                                  dfast, p, cb);
-            ct += (outA && cb) ? 1 : 0;
+            lp.count_caps(s, outA && cb);
             s.qy1 = face_land<R>(s.z1, s.lv1, s.n1, s.z0, s.lv0, s.n0, p.dx, p.rdx, dt, den,
                                  rden, dfast, p, cb);
-            ct += (outB && cb) ? 1 : 0;
+            lp.count_caps(s, outB && cb);
             if (K8) {
                 s.qd1_1 = face_land<R>(s.z1, s.lv1, s.n1, s.xz, s.xl, s.xn, p.ddiag, p.rddiag,
                                        dt, den, rden, dfast, p, cb);
-                ct += (outB && cb) ? 1 : 0;
+                lp.count_caps(s, outB && cb);
                 s.qd2_1 = face_land<R>(s.xq, s.xl1, s.xs, s.z0, s.lv0, s.n0, p.ddiag, p.rddiag,
                                        dt, den, rden, dfast, p, cb);
-                ct += (outB && cb) ? 1 : 0;
-            }
-            if (s.outcol) {
-                s.cap += ct;
+                lp.count_caps(s, outB && cb);
             }
         });
     } else {
@@ -592,13 +593,14 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                     /* rose in the previous step (h_old > stored hmax) and
                      * not in this one: store hmax = h_old and its tmax */
                     const int stop = sv.armed && s.h2 > s.phm && !(hnew > s.h2);
-                    s.v_rain += (double)A * (double)rain;
-                    /* the clamped volume is added only where some lane of
+                    s.v_rain += (double)rain;
+                    /* the clamped depth is added only where some lane of
                      * the wavefront clamps (adding +0 elsewhere would leave
-                     * the non-negative sum as it is) */
-                    if (wave_any(neg)) {
+                     * the non-negative sum as it is): a branch, not a select */
+                    if (wave_any(hraw < (R)0)) {
+                        FCA_COLD_PATH();
                         if (neg) {
-                            s.clamp += (double)(-hraw) * (double)A;
+                            s.clamp += (double)(-hraw);
                         }
                     }
                     s.stq = stop ? 3 : 1;
@@ -772,6 +774,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             s.pm2 = a.mask[(size_t)(rb + 1) * (size_t)nc + (uint32_t)s.c];
         }
     });
+    lp.strip_begun();
     /* rows before rb are not loaded; the ring holds them as outside the grid */
     land_rows = 0u;
     void_rows = ~0u;
@@ -835,7 +838,10 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         /* the row forms from row r's mask, fetched two rows ago, so that no
          * load of this row is waited for here */
         {
-            const uint64_t nonland = lp.non_land_lanes(&L::m0), land = lp.land_lanes(&L::m0);
+            /* a lane whose level or depth leaves the LAND form's range
+             * (|z + h| <= 2^20, |h| <= 2^14) counts as not LAND here */
+            const uint64_t nonland = lp.non_land_lanes(&L::m0) | lp.out_of_range_lanes();
+            const uint64_t land = lp.land_lanes(&L::m0);
             land_rows = (land_rows << 1) | (nonland == 0 ? 1u : 0u);
             void_rows = (void_rows << 1) | (land == 0 ? 1u : 0u);
         }
@@ -937,7 +943,7 @@ FCA_HD void walk_acc_init(LP &lp)
 {
     lp.each([&](WalkLane<R> &s, int) {
         s.v_rain = s.clamp = s.v_sea = s.v_bnd = 0.0;
-        s.cap = 0;
+        s.cap = s.capw = 0;
         s.dtc = (R)INFINITY;
         s.vmax = (R)0;
         s.vthr = (R)0;
@@ -952,6 +958,8 @@ FCA_HD void walk_acc_finish(LP &lp, const PhysK<R> &p)
 {
     lp.each([&](WalkLane<R> &s, int) {
         s.dtc = (s.vmax > (R)0) ? (p.alpha * p.dx) / s.vmax : (R)INFINITY;
+        s.v_rain *= (double)p.A;
+        s.clamp *= (double)p.A;
     });
 }
 

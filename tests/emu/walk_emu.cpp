@@ -50,6 +50,23 @@ struct CpuLanes {
             s[l].*dst = tmp[l];
         }
     }
+    void strip_begun() {}
+    /* the capped faces of written columns where c holds, per lane */
+    void count_caps(WalkLane<R> &t, bool c)
+    {
+        t.cap += (c && t.outcol) ? 1 : 0;
+    }
+    /* the device's range test of the LAND form's rows (float) */
+    uint64_t out_of_range_lanes()
+    {
+        uint64_t b = 0;
+        if (sizeof(R) == 4) {
+            for (int l = 0; l < W; l++) {
+                b |= (uint64_t)(!(fabs(s[l].lv0) <= 0x1p20) || !(fabs(s[l].h0) <= 0x1p14)) << (l & 63);
+            }
+        }
+        return b;
+    }
     long form_rows[3];
     void count_form(int form)
     {
