@@ -796,6 +796,97 @@ FCA_HDI R face_land(R za, R la, R na, R zb, R lb, R nb, R d, R rd, R dt, R den, 
     return q;
 }
 
+/* Two faces of the LAND form at once (the east face of row r and the face
+ * from row r-1 to row r, or the two diagonals between them; both of length
+ * d): on each face face_land's value, bit for bit, with the two faces'
+ * arithmetic in the same basic blocks and one uniform branch where
+ * face_land takes one per face (no live face, a critical speed to take, a
+ * certificate to redo), so that the compiler can interleave the two faces'
+ * independent chains of dependent instructions. */
+template <class R>
+FCA_HDI void face_land_pair(const R (&za)[2], const R (&la)[2], const R (&na)[2],
+                            const R (&zb)[2], const R (&lb)[2], const R (&nb)[2], R d, R rd,
+                            R dt, R den, R rden, int den_fast, const PhysK<R> &p, R (&q)[2],
+                            bool (&cap)[2])
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (sizeof(R) == 4) {
+        float dL[2], adL[2], hf[2], v[2], qm[2];
+        bool up[2], live[2];
+        uint64_t mlive[2], mfix[2];
+        for (int k = 0; k < 2; k++) {
+            dL[k] = (float)la[k] - (float)lb[k];
+            up[k] = dL[k] > 0.0f;
+            adL[k] = __builtin_fabsf(dL[k]);
+            hf[k] = (up[k] ? (float)la[k] : (float)lb[k]) -
+                    (((float)za[k] > (float)zb[k]) ? (float)za[k] : (float)zb[k]);
+            live[k] = !(adL[k] < (float)p.tau_l) && !(hf[k] < (float)p.tau_h);
+            mlive[k] = lane_mask(!(adL[k] < (float)p.tau_l)) & lane_mask(!(hf[k] < (float)p.tau_h));
+            cap[k] = false;
+        }
+        if ((mlive[0] | mlive[1]) == 0) {
+            q[0] = q[1] = (R)0;
+            return;
+        }
+        /* as in face_land: dead lanes compute on their own inputs, masked
+         * at the end; mfix collects the lanes whose certificate fails */
+        for (int k = 0; k < 2; k++) {
+            mfix[k] = den_fast ? 0ull : ~0ull;
+            const float nf = 0.5f * ((float)na[k] + (float)nb[k]);
+            const float q0 = adL[k] * (float)rd;
+            const float sl = __builtin_fmaf(__builtin_fmaf(-q0, (float)d, adL[k]), (float)rd, q0);
+            const double y1 = cbrt_sq_newton(hf[k], (double)hf[k] * (double)hf[k]);
+            mfix[k] |= lane_mask(near_float_midpoint(y1));
+            v[k] = rcp_in_range(nf) * (float)y1 * sqrt_in_range(sl);
+        }
+        /* the critical speed of a face where some live lane's v is not
+         * clearly below it (face_land's test), face by face */
+        for (int k = 0; k < 2; k++) {
+            if ((lane_mask(!(v[k] * v[k] < hf[k] * (float)p.gk)) & mlive[k]) != 0) {
+                bool cfix = false;
+                const float vcr = crit_speed_cert((float)p.g, hf[k], cfix);
+                mfix[k] |= lane_mask(cfix);
+                if (v[k] > vcr) {
+                    v[k] = vcr;
+                }
+            }
+        }
+        for (int k = 0; k < 2; k++) {
+            float qmag = v[k] * hf[k] * (float)p.w;
+            const float x = adL[k] * (float)p.A;
+            const float c0 = x * (float)rden;
+            const float qcap = __builtin_fmaf(__builtin_fmaf(-c0, (float)den, x), (float)rden, c0);
+            cap[k] = live[k] && qcap < qmag;
+            qmag = (qcap < qmag) ? qcap : qmag;
+            qm[k] = up[k] ? qmag : -qmag;
+        }
+        if (((mfix[0] & mlive[0]) | (mfix[1] & mlive[1])) != 0) {
+            FCA_COLD_PATH();
+            const uint64_t lane_bit = 1ull << __builtin_amdgcn_mbcnt_hi(
+                                          ~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+            for (int k = 0; k < 2; k++) {
+                if ((mfix[k] & mlive[k] & lane_bit) != 0) {
+                    int cx;
+                    qm[k] = (float)face_land_exact<R>(za[k], la[k], na[k], zb[k], lb[k], nb[k], d,
+                                                      rd, dt, den, rden, den_fast, p, cx);
+                    cap[k] = cx != 0;
+                }
+            }
+        }
+        for (int k = 0; k < 2; k++) {
+            /* a dead face is +0, as face_land_exact (a lane the exact path
+             * took is live) */
+            q[k] = live[k] ? (R)qm[k] : (R)0;
+        }
+        return;
+    }
+#endif
+    for (int k = 0; k < 2; k++) {
+        q[k] = face_land<R>(za[k], la[k], na[k], zb[k], lb[k], nb[k], d, rd, dt, den, rden,
+                            den_fast, p, cap[k]);
+    }
+}
+
 /* Open-boundary virtual face of a LAND edge cell: z_b = z_a - S_bnd*dx,
  * h_b = 0, n_b = n_a, d = dx; inflow clipped to 0 (model.c:166-177). */
 template <class R>

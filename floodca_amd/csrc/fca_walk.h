@@ -338,20 +338,27 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
             lp.pull_right(&L::xs, &L::n1);
         }
         lp.each([&](L &s, int) {
-            bool cb;
-            s.qx0 = face_land<R>(s.z0, s.lv0, s.n0, s.xz, s.xl, s.xn, p.dx, p.rdx, dt, den, rden,
-                                 dfast, p, cb);
-            lp.count_caps(s, outA && cb);
-            s.qy1 = face_land<R>(s.z1, s.lv1, s.n1, s.z0, s.lv0, s.n0, p.dx, p.rdx, dt, den,
-                                 rden, dfast, p, cb);
-            lp.count_caps(s, outB && cb);
+            /* the east face of row r and the face from row r-1 to row r
+             * together (and the two diagonals), so their work interleaves */
+            R q[2];
+            bool cb[2];
+            {
+                const R za[2] = {s.z0, s.z1}, la[2] = {s.lv0, s.lv1}, na[2] = {s.n0, s.n1};
+                const R zb[2] = {s.xz, s.z0}, lb[2] = {s.xl, s.lv0}, nb[2] = {s.xn, s.n0};
+                face_land_pair<R>(za, la, na, zb, lb, nb, p.dx, p.rdx, dt, den, rden, dfast, p, q,
+                                  cb);
+            }
+            s.qx0 = q[0], s.qy1 = q[1];
+            lp.count_caps(s, outA && cb[0]);
+            lp.count_caps(s, outB && cb[1]);
             if (K8) {
-                s.qd1_1 = face_land<R>(s.z1, s.lv1, s.n1, s.xz, s.xl, s.xn, p.ddiag, p.rddiag,
-                                       dt, den, rden, dfast, p, cb);
-                lp.count_caps(s, outB && cb);
-                s.qd2_1 = face_land<R>(s.xq, s.xl1, s.xs, s.z0, s.lv0, s.n0, p.ddiag, p.rddiag,
-                                       dt, den, rden, dfast, p, cb);
-                lp.count_caps(s, outB && cb);
+                const R za[2] = {s.z1, s.xq}, la[2] = {s.lv1, s.xl1}, na[2] = {s.n1, s.xs};
+                const R zb[2] = {s.xz, s.z0}, lb[2] = {s.xl, s.lv0}, nb[2] = {s.xn, s.n0};
+                face_land_pair<R>(za, la, na, zb, lb, nb, p.ddiag, p.rddiag, dt, den, rden, dfast,
+                                  p, q, cb);
+                s.qd1_1 = q[0], s.qd2_1 = q[1];
+                lp.count_caps(s, outB && cb[0]);
+                lp.count_caps(s, outB && cb[1]);
             }
         });
     } else {
