@@ -188,3 +188,31 @@ def test_hmax_below_the_initial_depth(prec, hyeto):
             assert np.array_equal(getattr(go, k), getattr(ge, k)), (k, skip)
         if skip and hyeto == A.FCA_HYETO_TRIANGLE:
             assert skipped[0] > 0  # a 0.5 mm storm over 30 min: dry tiles go quiet
+
+
+@pytest.mark.parametrize("K", [4, 8])
+def test_out_of_range_rows_take_the_general_form(K):
+    """An all-LAND grid whose middle rows stand on a 2^21 m plateau (|z + h|
+    beyond the LAND form's 2^20) and hold one 2^15 m column of water (|h|
+    beyond 2^14): those rows, and the four after each, take the general form
+    (more general rows than on the same grid without them), and the result is
+    the oracle's bit for bit."""
+    rows = []
+    for tall in (False, True):
+        a, g, cfg = make_case(48, 150, K=K, steps=40, T_sim=1e9, seed=40 + K, land_frac=1.0)
+        g.mask[...] = A.FCA_LAND
+        if tall:
+            g.z[20:24, :] += np.float32(2.0 ** 21)
+            g.h[33, 70] = np.float32(2.0 ** 15)
+            g.hmax[33, 70] = g.h[33, 70]
+        o = Oracle("f32")
+        go, ge = g.copy(), g.copy()
+        acc_o, log_o = o.run(go, cfg, log_cap=100000)
+        forms = []
+        acc_e, log_e = emu_run(ge, cfg, o.acc_init(ge, cfg), "f32", strip_rows=24, forms=forms)
+        for k in ("h", "hmax", "tmax"):
+            assert np.array_equal(getattr(go, k), getattr(ge, k)), (tall, k)
+        assert [r.dt for r in log_o] == [r.dt for r in log_e]
+        assert acc_o.cap_active == acc_e.cap_active
+        rows.append(forms)
+    assert rows[1][0] > rows[0][0] and rows[1][1] < rows[0][1], rows

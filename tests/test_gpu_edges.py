@@ -7,7 +7,9 @@ tests: FP32 bit for bit (h, hmax, tmax, dt sequence), FP64 within 1e-12 m.
   live, no depth moves, and every step is dt_max (model.c:150-152);
 * a single cell, with the open and the closed boundary;
 * deep water (tens of metres) and depths straddling tau_h;
-* an all-LAND grid with walls, where the LAND row form covers the interior.
+* an all-LAND grid with walls, where the LAND row form covers the interior;
+* LAND rows beyond the LAND form's range (a 2^21 m plateau, a 2^15 m column
+  of water), which the walk sends through the general form.
 """
 import numpy as np
 import pytest
@@ -71,3 +73,17 @@ def test_all_land_with_walls(K):
     assert (g.mask == A.FCA_LAND).mean() > 0.9
     print(f"case all land K={K}", flush=True)
     assert_parity("f32", *run_both(a, g, cfg, "f32", strip=16))
+
+
+@pytest.mark.parametrize("K", [4, 8])
+def test_rows_beyond_the_land_form_range(K):
+    """Levels past 2^20 m and a depth past 2^14 m on an all-LAND grid: the
+    LAND form's fast face arithmetic assumes neither (its row test sends such
+    rows to the general form), and the step stays the oracle's bit for bit."""
+    a, g, cfg = make_case(48, 150, K=K, steps=40, T_sim=1e9, seed=40 + K, land_frac=1.0)
+    g.mask[...] = A.FCA_LAND
+    g.z[20:24, :] += np.float32(2.0 ** 21)
+    g.h[33, 70] = np.float32(2.0 ** 15)
+    g.hmax[33, 70] = g.h[33, 70]
+    print(f"case beyond LAND range K={K}", flush=True)
+    assert_parity("f32", *run_both(a, g, cfg, "f32", strip=24))
