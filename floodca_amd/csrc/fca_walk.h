@@ -124,6 +124,10 @@ struct WalkLane {
     /* stage D: largest |level difference| per face length, and whether the
      * exact speed is needed */
     R dax, dad;
+    /* stage D of the LAND form: levels z + h (new depth) of rows r-2, r-3,
+     * r-4 on this lane, and row r-3's level pulled from the east and west */
+    R dl2, dl3, dl4, dle, dlw;
+
     int dneed;
 };
 
@@ -625,6 +629,21 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
 
     /* ---- D: next step's stable time step on row r-3 -------------------- */
     if (sv.need_cand && qD >= w.r_out0 && qD < w.r_out1 && qD < nr) {
+      if (LANDF) {
+        /* LAND form: each lane forms its own levels once and the neighbours'
+         * are pulled as levels (the same sums, formed by their own lanes) */
+        lp.each([&](L &s, int) {
+            s.dl2 = s.z2 + s.hn2, s.dl3 = s.z3 + s.hn3, s.dl4 = s.z4 + s.hn4;
+        });
+        lp.pull_right(&L::dle, &L::dl3);
+        lp.pull_left(&L::dlw, &L::dl3);
+        if (K8) {
+            lp.pull_right(&L::dz1, &L::dl2);
+            lp.pull_left(&L::dz2, &L::dl2);
+            lp.pull_right(&L::dz3, &L::dl4);
+            lp.pull_left(&L::dz4, &L::dl4);
+        }
+      } else {
         lp.pull_right(&L::xz, &L::z3);
         lp.pull_right(&L::xh, &L::hn3);
         lp.pull_left(&L::xq, &L::z3);
@@ -649,13 +668,14 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 lp.pull_left(&L::dm4, &L::m4);
             }
         }
+      }
         lp.each([&](L &s, int) {
             R la, ax = (R)0, ad = (R)0, s_hi;
             s.dneed = 0;
             if (!s.outcol || (!LANDF && s.m3 != FCA_LAND) || s.hn3 < p.tau_h) {
                 return;
             }
-            la = s.z3 + s.hn3;
+            la = LANDF ? s.dl3 : s.z3 + s.hn3;
             /* largest |level difference| per face length: max_k RN(a_k/d)
              * = RN(max_k a_k / d) as rounding is monotonic, so one
              * division per length replaces one per neighbour. The LAND form
@@ -663,9 +683,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
              * only in the sign of a zero maximum, and a zero slope gives the
              * floor speed either way. */
 #define FCA_LEVEL_DIFF(cond, m_, z_, h_, acc_)                                \
-    if (LANDF) {                                                                \
-        acc_ = fmax_r(acc_, fabs_r(la - ((z_) + (h_))));                        \
-    } else if ((cond) && (m_) != FCA_WALL) {                                    \
+    if (!LANDF && (cond) && (m_) != FCA_WALL) {                                 \
         R dl_ = la - level_of<R>((m_), (z_), (h_), p);                          \
         if (dl_ < (R)0) {                                                       \
             dl_ = -dl_;                                                         \
@@ -674,6 +692,12 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
             acc_ = dl_;                                                         \
         }                                                                       \
     }
+            if (LANDF) {
+                ax = fmax_r(ax, fabs_r(la - s.dle));
+                ax = fmax_r(ax, fabs_r(la - s.dlw));
+                ax = fmax_r(ax, fabs_r(la - s.dl2));
+                ax = fmax_r(ax, fabs_r(la - s.dl4));
+            }
             FCA_LEVEL_DIFF(s.c + 1 < nc, s.xm, s.xz, s.xh, ax)
             FCA_LEVEL_DIFF(s.c > 0, s.xm2, s.xq, s.xr, ax)
             FCA_LEVEL_DIFF(qD + 1 < nr, s.m2, s.z2, s.hn2, ax)
@@ -686,6 +710,12 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 if (dl_ > ax) {
                     ax = dl_;
                 }
+            }
+            if (K8 && LANDF) {
+                ad = fmax_r(ad, fabs_r(la - s.dz1));
+                ad = fmax_r(ad, fabs_r(la - s.dz2));
+                ad = fmax_r(ad, fabs_r(la - s.dz3));
+                ad = fmax_r(ad, fabs_r(la - s.dz4));
             }
             if (K8) {
                 FCA_LEVEL_DIFF(qD + 1 < nr && s.c + 1 < nc, s.dm1, s.dz1, s.dh1, ad)
