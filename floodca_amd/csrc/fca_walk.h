@@ -330,6 +330,22 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
     const int outA = r >= w.r_out0 && r < w.r_out1;    /* row r is written here */
     const int qB = r - 1, qC = r - 2, qD = r - 3;
     const int outB = qB >= w.r_out0 && qB < w.r_out1;
+    const int dD = sv.need_cand && qD >= w.r_out0 && qD < w.r_out1 && qD < nr;
+
+    /* ---- D, first part (LAND form, K = 4): the level differences of row
+     * r-3 that do not wait for this row's update -- east, west and north,
+     * from depths of earlier rows -- are formed before the faces, so their
+     * work can share the faces' basic blocks; the south one follows stage C.
+     * (A maximum of magnitudes does not depend on its order.) */
+    if (LANDF && !K8 && dD) {
+        lp.each([&](L &s, int) { s.dl3 = s.z3 + s.hn3, s.dl4 = s.z4 + s.hn4; });
+        lp.pull_right(&L::dle, &L::dl3);
+        lp.pull_left(&L::dlw, &L::dl3);
+        lp.each([&](L &s, int) {
+            s.dax = fmax_r(fmax_r(fmax_r((R)0, fabs_r(s.dl3 - s.dle)), fabs_r(s.dl3 - s.dlw)),
+                           fabs_r(s.dl3 - s.dl4));
+        });
+    }
 
     /* ---- A: the faces that touch row r ---------------------------------- */
     if (LANDF) {
@@ -628,8 +644,10 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
     }
 
     /* ---- D: next step's stable time step on row r-3 -------------------- */
-    if (sv.need_cand && qD >= w.r_out0 && qD < w.r_out1 && qD < nr) {
-      if (LANDF) {
+    if (dD) {
+      if (LANDF && !K8) {
+        lp.each([&](L &s, int) { s.dl2 = s.z2 + s.hn2; });
+      } else if (LANDF) {
         /* LAND form: each lane forms its own levels once and the neighbours'
          * are pulled as levels (the same sums, formed by their own lanes) */
         lp.each([&](L &s, int) {
@@ -692,7 +710,9 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
             acc_ = dl_;                                                         \
         }                                                                       \
     }
-            if (LANDF) {
+            if (LANDF && !K8) {
+                ax = fmax_r(s.dax, fabs_r(la - s.dl2));
+            } else if (LANDF) {
                 ax = fmax_r(ax, fabs_r(la - s.dle));
                 ax = fmax_r(ax, fabs_r(la - s.dlw));
                 ax = fmax_r(ax, fabs_r(la - s.dl2));
