@@ -64,6 +64,13 @@
 #define FCA_TILE_OUT(width) ((width) - 2 * FCA_HALO)
 #define FCA_OUTC FCA_TILE_OUT(FCA_WAVE)
 
+/* Logarithms of a cell's depth and roughness in the speed test of stage D
+ * (speed_logs, below). */
+template <class R>
+struct SpeedLogs {
+    R lh, lcr, lnn;
+};
+
 /* Inputs of one launch of the fused step. */
 template <class R>
 struct WalkArgs {
@@ -127,6 +134,8 @@ struct WalkLane {
     /* stage D of the LAND form: levels z + h (new depth) of rows r-2, r-3,
      * r-4 on this lane, and row r-3's level pulled from the east and west */
     R dl2, dl3, dl4, dle, dlw;
+    R cnet; /* stage C of the LAND form (K = 4): net after the E and W terms */
+    SpeedLogs<R> dlg; /* its speed test's logarithms (speed_logs) */
 
     int dneed;
 };
@@ -267,6 +276,48 @@ FCA_HDI int speed_may_exceed_fast(R h, R n, R s_hi, int flat, R thr, R lthr, R l
     return speed_may_exceed<R>(h, n, s_hi, flat, thr, g);
 }
 
+/* speed_may_exceed_fast in two parts: the logarithms that depend on the
+ * cell's depth and roughness alone (speed_logs), which a walk may form
+ * before the cell's slope is known, and the test given its slope bound
+ * (speed_may_exceed_logs); the same operations, so the same result. */
+template <class R>
+FCA_HDI SpeedLogs<R> speed_logs(R h, R n, R lgg)
+{
+    SpeedLogs<R> o;
+    o.lh = o.lcr = o.lnn = (R)0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (sizeof(R) == 4) {
+        const float nn = (n < 1e-12f) ? 1e-12f : (float)n;
+        o.lh = (R)__builtin_amdgcn_logf((float)h);
+        o.lcr = (R)__builtin_fmaf((float)o.lh, 0.5f, (float)lgg);
+        o.lnn = (R)__builtin_amdgcn_logf(nn);
+    }
+#endif
+    (void)h, (void)n, (void)lgg;
+    return o;
+}
+template <class R>
+FCA_HDI int speed_may_exceed_logs(R h, R n, R s_hi, int flat, R thr, R lthr,
+                                  const SpeedLogs<R> &lg, R g)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (sizeof(R) == 4) {
+        if (flat) {
+            return !(thr >= (R)1e-6);
+        }
+        {
+            const float l = __builtin_fmaf(
+                (float)lg.lh, 2.0f / 3.0f,
+                __builtin_fmaf(__builtin_amdgcn_logf((float)s_hi), 0.5f, -(float)lg.lnn));
+            const float lm = (l < (float)lg.lcr) ? l : (float)lg.lcr;
+            return !(lm < (float)lthr && __builtin_fminf((float)h, (float)s_hi) >= 0x1p-126f);
+        }
+    }
+#endif
+    (void)lthr, (void)lg;
+    return speed_may_exceed<R>(h, n, s_hi, flat, thr, g);
+}
+
 /* (1/2) lg g for speed_may_exceed_fast (device, float). */
 template <class R>
 FCA_HDI R speed_log_half_g(R g)
@@ -344,6 +395,21 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
         lp.each([&](L &s, int) {
             s.dax = fmax_r(fmax_r(fmax_r((R)0, fabs_r(s.dl3 - s.dle)), fabs_r(s.dl3 - s.dlw)),
                            fabs_r(s.dl3 - s.dl4));
+            s.dlg = speed_logs<R>(s.hn3, s.n3, a.lgg);
+        });
+    }
+    /* ---- C, first part (LAND form, K = 4): the east and west terms of row
+     * r-2's net inflow, whose limiters are known from the previous row, in
+     * the reference's order (net = -E + W, then -S + N after stage B) */
+    if (LANDF && !K8) {
+        lp.pull_right(&L::xw, &L::s2);
+        lp.pull_left(&L::xy, &L::s2);
+        lp.pull_left(&L::xa, &L::qx2);
+        lp.each([&](L &s, int) {
+            R net = (R)0;
+            net -= s.qx2 * ((s.qx2 >= (R)0) ? s.s2 : s.xw);
+            net += s.xa * ((s.xa >= (R)0) ? s.xy : s.s2);
+            s.cnet = net;
         });
     }
 
@@ -500,9 +566,11 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
     }
 
     /* ---- C: update row r-2 --------------------------------------------- */
-    lp.pull_right(&L::xw, &L::s2);
-    lp.pull_left(&L::xy, &L::s2);
-    lp.pull_left(&L::xa, &L::qx2);
+    if (!(LANDF && !K8)) {
+        lp.pull_right(&L::xw, &L::s2);
+        lp.pull_left(&L::xy, &L::s2);
+        lp.pull_left(&L::xa, &L::qx2);
+    }
     if (K8) {
         lp.pull_right(&L::dz1, &L::s1); /* s of (q+1, c+1) */
         lp.pull_left(&L::dz2, &L::s1);  /* s of (q+1, c-1) */
@@ -532,14 +600,16 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 return;
             }
             /* E, W, S, N[, SE, NW, SW, NE] in the reference's order */
-            if (LANDF || s.c + 1 < nc) {
+            if (LANDF && !K8) {
+                net = s.cnet;
+            } else if (LANDF || s.c + 1 < nc) {
                 q = s.qx2 * ((s.qx2 >= (R)0) ? s.s2 : s.xw);
                 net -= q;
                 if (!LANDF && land && s.xm3 == FCA_SEA && q > (R)0) {
                     sea += (double)dt * (double)q;
                 }
             }
-            if (LANDF || s.c > 0) {
+            if (!(LANDF && !K8) && (LANDF || s.c > 0)) {
                 q = s.xa * ((s.xa >= (R)0) ? s.xy : s.s2);
                 net += q;
                 if (!LANDF && land && s.xm4 == FCA_SEA && q < (R)0) {
@@ -753,8 +823,12 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 s_hi = (sd > s_hi) ? sd : s_hi;
             }
             s_hi = s_hi * ((R)1 + (R)0x1p-20);
-            s.dneed = speed_may_exceed_fast<R>(s.hn3, s.n3, s_hi, ax == (R)0 && ad == (R)0,
-                                               s.vthr, s.lthr, a.lgg, p.g);
+            s.dneed = (LANDF && !K8)
+                          ? speed_may_exceed_logs<R>(s.hn3, s.n3, s_hi, ax == (R)0 && ad == (R)0,
+                                                     s.vthr, s.lthr, s.dlg, p.g)
+                          : speed_may_exceed_fast<R>(s.hn3, s.n3, s_hi,
+                                                     ax == (R)0 && ad == (R)0, s.vthr, s.lthr,
+                                                     a.lgg, p.g);
         });
         if (lp.any(&L::dneed)) {
             lp.each([&](L &s, int) {
