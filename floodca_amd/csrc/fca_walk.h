@@ -382,13 +382,16 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
     const int qB = r - 1, qC = r - 2, qD = r - 3;
     const int outB = qB >= w.r_out0 && qB < w.r_out1;
     const int dD = sv.need_cand && qD >= w.r_out0 && qD < w.r_out1 && qD < nr;
+    /* the LAND form for K = 4 in float forms a row's independent work first
+     * (below); in double the longer live ranges cost more than they save */
+    constexpr bool HOIST = LANDF && !K8 && sizeof(R) == 4;
 
     /* ---- D, first part (LAND form, K = 4): the level differences of row
      * r-3 that do not wait for this row's update -- east, west and north,
      * from depths of earlier rows -- are formed before the faces, so their
      * work can share the faces' basic blocks; the south one follows stage C.
      * (A maximum of magnitudes does not depend on its order.) */
-    if (LANDF && !K8 && dD) {
+    if (HOIST && dD) {
         lp.each([&](L &s, int) { s.dl3 = s.z3 + s.hn3, s.dl4 = s.z4 + s.hn4; });
         lp.pull_right(&L::dle, &L::dl3);
         lp.pull_left(&L::dlw, &L::dl3);
@@ -401,7 +404,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
     /* ---- C, first part (LAND form, K = 4): the east and west terms of row
      * r-2's net inflow, whose limiters are known from the previous row, in
      * the reference's order (net = -E + W, then -S + N after stage B) */
-    if (LANDF && !K8) {
+    if (HOIST) {
         lp.pull_right(&L::xw, &L::s2);
         lp.pull_left(&L::xy, &L::s2);
         lp.pull_left(&L::xa, &L::qx2);
@@ -566,7 +569,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
     }
 
     /* ---- C: update row r-2 --------------------------------------------- */
-    if (!(LANDF && !K8)) {
+    if (!HOIST) {
         lp.pull_right(&L::xw, &L::s2);
         lp.pull_left(&L::xy, &L::s2);
         lp.pull_left(&L::xa, &L::qx2);
@@ -600,7 +603,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 return;
             }
             /* E, W, S, N[, SE, NW, SW, NE] in the reference's order */
-            if (LANDF && !K8) {
+            if (HOIST) {
                 net = s.cnet;
             } else if (LANDF || s.c + 1 < nc) {
                 q = s.qx2 * ((s.qx2 >= (R)0) ? s.s2 : s.xw);
@@ -609,7 +612,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                     sea += (double)dt * (double)q;
                 }
             }
-            if (!(LANDF && !K8) && (LANDF || s.c > 0)) {
+            if (!HOIST && (LANDF || s.c > 0)) {
                 q = s.xa * ((s.xa >= (R)0) ? s.xy : s.s2);
                 net += q;
                 if (!LANDF && land && s.xm4 == FCA_SEA && q < (R)0) {
@@ -715,7 +718,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
 
     /* ---- D: next step's stable time step on row r-3 -------------------- */
     if (dD) {
-      if (LANDF && !K8) {
+      if (HOIST) {
         lp.each([&](L &s, int) { s.dl2 = s.z2 + s.hn2; });
       } else if (LANDF) {
         /* LAND form: each lane forms its own levels once and the neighbours'
@@ -780,7 +783,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
             acc_ = dl_;                                                         \
         }                                                                       \
     }
-            if (LANDF && !K8) {
+            if (HOIST) {
                 ax = fmax_r(s.dax, fabs_r(la - s.dl2));
             } else if (LANDF) {
                 ax = fmax_r(ax, fabs_r(la - s.dle));
@@ -823,7 +826,7 @@ FCA_HDI void walk_row(LP &lp, const WalkArgs<R> &a, const WalkRow &w)
                 s_hi = (sd > s_hi) ? sd : s_hi;
             }
             s_hi = s_hi * ((R)1 + (R)0x1p-20);
-            s.dneed = (LANDF && !K8)
+            s.dneed = HOIST
                           ? speed_may_exceed_logs<R>(s.hn3, s.n3, s_hi, ax == (R)0 && ad == (R)0,
                                                      s.vthr, s.lthr, s.dlg, p.g)
                           : speed_may_exceed_fast<R>(s.hn3, s.n3, s_hi,
