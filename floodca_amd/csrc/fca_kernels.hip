@@ -483,8 +483,11 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
 /* Wavefronts per SIMD the walk is compiled for, each the most that compiles
  * without spilling to scratch. Float, one column per work-item: the K=4 walk
  * without the quiet-tile bookkeeping fits four (at most 128 VGPRs), whose
- * fourth wavefront hides row-load latency; the others need more registers
- * and run three. Float, two columns per work-item: about 170 VGPRs for K=4
+ * fourth wavefront hides row-load latency, and so do both K=8 walks with
+ * their row accesses on a scalar base (121-126 VGPRs; 136-140 with plain
+ * global accesses, which ran three); the K=4 walk with the bookkeeping runs
+ * three (134 VGPRs; on a scalar base it fits four, but ran up to 13% slower
+ * on wet grids). Float, two columns per work-item: about 170 VGPRs for K=4
  * (three wavefronts, FCA_CPL2_WAVES) and more with K=8 or the bookkeeping
  * (two). Double, one column per work-item: three for K=4, two for K=8 (its
  * two-column form is not built). */
@@ -495,7 +498,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
 #define FCA_FUSED_WAVES(K8, QT, CPL) ((K8) ? 2 : 3)
 #else
 #define FCA_FUSED_WAVES(K8, QT, CPL) \
-    ((CPL) == 2 ? ((K8) || (QT) ? 2 : FCA_CPL2_WAVES) : ((K8) || (QT) ? 3 : 4))
+    ((CPL) == 2 ? ((K8) || (QT) ? 2 : FCA_CPL2_WAVES) : ((QT) && !(K8) ? 3 : 4))
 #endif
 #define FCA_FUSED_OCCUPANCY(K8, QT, CPL) \
     __attribute__((amdgpu_waves_per_eu(FCA_FUSED_WAVES(K8, QT, CPL))))

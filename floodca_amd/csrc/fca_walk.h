@@ -141,15 +141,42 @@ struct WalkLane {
 };
 
 /* Loads the element at byte offset boff of a row: a uniform row address plus
- * a 32-bit lane offset. */
-template <class T>
+ * a 32-bit lane offset. Every caller passes a wavefront-uniform row. On the
+ * device there are two forms. With SB false, a plain global access: the
+ * compiler keeps each field's 64-bit lane address (base + boff) in two
+ * vector registers and adds the row's offset with one vector add per access.
+ * With SB true, the row's address is pinned to two scalar registers and boff
+ * passed as the access's 32-bit vector offset: no vector register holds an
+ * address, at the cost of scalar work per row. The plain form is the faster
+ * wherever the walk fits its occupancy anyway; the scalar base frees the
+ * registers the float K = 8 walk needs to run four wavefronts per SIMD. */
+template <bool SB, class T>
 FCA_HDI T ld_row(const T *row, uint32_t boff)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (SB) {
+        typedef const __attribute__((address_space(1))) char gchar;
+        typedef const __attribute__((address_space(1))) T gT;
+        __asm__("" : "+s"(row));
+        __asm__("" : "+v"(boff));
+        return *(gT *)((gchar *)row + boff);
+    }
+#endif
     return *(const T *)((const char *)row + boff);
 }
-template <class T>
+template <bool SB, class T>
 FCA_HDI void st_row(T *row, uint32_t boff, T v)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (SB) {
+        typedef __attribute__((address_space(1))) char gchar;
+        typedef __attribute__((address_space(1))) T gT;
+        __asm__("" : "+s"(row));
+        __asm__("" : "+v"(boff));
+        *(gT *)((gchar *)row + boff) = v;
+        return;
+    }
+#endif
     *(T *)((char *)row + boff) = v;
 }
 
@@ -863,6 +890,10 @@ template <class R, int K8, int QT, class LP>
 FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int c_first)
 {
     typedef WalkLane<R> L;
+    /* row accesses with a scalar base for the float K = 8 walk of one
+     * column per work-item: the vector registers the plain form spends on
+     * addresses are what keeps it from four wavefronts per SIMD */
+    constexpr bool RF = K8 && sizeof(R) == 4 && LP::WIDTH == 64;
     const int nr = a.nr, nc = a.nc;
     const int rb = (r_out0 - 3 > 0) ? r_out0 - 3 : 0;
     const int re = (r_out1 + 2 < nr + 2) ? r_out1 + 2 : nr + 2; /* inclusive */
@@ -899,9 +930,9 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             const size_t ro = (size_t)rb * (size_t)nc;
             s.pm1 = a.mask[ro + (uint32_t)s.c];
             if (s.pm1 == FCA_LAND) {
-                s.pz = ld_row(a.z + ro, s.boff);
-                s.ph = ld_row(a.h_in + ro, s.boff);
-                s.pn = ld_row(a.n + ro, s.boff);
+                s.pz = ld_row<RF>(a.z + ro, s.boff);
+                s.ph = ld_row<RF>(a.h_in + ro, s.boff);
+                s.pn = ld_row<RF>(a.n + ro, s.boff);
             }
         }
         if (s.colok && rb + 1 < nr) {
@@ -946,25 +977,25 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             lp.loads_settled();
             lp.each([&](L &s, int) {
                 if (s.stq) {
-                    st_row(hos, s.boff, s.hn3);
+                    st_row<RF>(hos, s.boff, s.hn3);
                     if (s.stq & 2) {
-                        st_row(hms, s.boff, s.hst);
-                        st_row(tms, s.boff, tval);
+                        st_row<RF>(hms, s.boff, s.hst);
+                        st_row<RF>(tms, s.boff, tval);
                     }
                     s.stq = 0;
                 }
                 s.pm1 = s.pm2;
                 if (s.colok && ok1 && s.pm1 == FCA_LAND) {
-                    s.pz = ld_row(zr, s.boff), s.ph = ld_row(hr, s.boff);
-                    s.pn = ld_row(nrw, s.boff);
+                    s.pz = ld_row<RF>(zr, s.boff), s.ph = ld_row<RF>(hr, s.boff);
+                    s.pn = ld_row<RF>(nrw, s.boff);
                 }
-                s.pm2 = (s.colok && ok2) ? (int)mr[(uint32_t)s.c] : FCA_NONE;
+                s.pm2 = (s.colok && ok2) ? (int)ld_row<RF>(mr, (uint32_t)s.c) : FCA_NONE;
                 /* cr and hmax of row r-1 (m1), updated one row from now;
                  * hmax is read before that update writes it */
                 if (s.colok && okq && s.m1 == FCA_LAND) {
-                    s.pcr1 = ld_row(crr, s.boff);
+                    s.pcr1 = ld_row<RF>(crr, s.boff);
                     if (s.outcol && outq) {
-                        s.phm1 = ld_row(hmr, s.boff);
+                        s.phm1 = ld_row<RF>(hmr, s.boff);
                     }
                 }
             });
@@ -1060,10 +1091,10 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         R *hos = a.h_out + ros, *hms = a.hmax + ros, *tms = a.tmax + ros;
         lp.each([&](L &s, int) {
             if (s.stq) {
-                st_row(hos, s.boff, s.hn3);
+                st_row<RF>(hos, s.boff, s.hn3);
                 if (s.stq & 2) {
-                    st_row(hms, s.boff, s.hst);
-                    st_row(tms, s.boff, a.sv.tmax_prev);
+                    st_row<RF>(hms, s.boff, s.hst);
+                    st_row<RF>(tms, s.boff, a.sv.tmax_prev);
                 }
                 s.stq = 0;
             }
