@@ -489,13 +489,14 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
  * three (134 VGPRs; on a scalar base it fits four, but ran up to 13% slower
  * on wet grids). Float, two columns per work-item: about 170 VGPRs for K=4
  * (three wavefronts, FCA_CPL2_WAVES) and more with K=8 or the bookkeeping
- * (two). Double, one column per work-item: three for K=4, two for K=8 (its
- * two-column form is not built). */
+ * (two). Double, one column per work-item: three for K=4 (163-165 VGPRs) and
+ * for K=8 with its row accesses on a scalar base (164-168; 179-184 with
+ * plain global accesses, which ran two); its two-column form is not built. */
 #ifndef FCA_CPL2_WAVES
 #define FCA_CPL2_WAVES 3
 #endif
 #ifdef FCA_DOUBLE
-#define FCA_FUSED_WAVES(K8, QT, CPL) ((K8) ? 2 : 3)
+#define FCA_FUSED_WAVES(K8, QT, CPL) 3
 #else
 #define FCA_FUSED_WAVES(K8, QT, CPL) \
     ((CPL) == 2 ? ((K8) || (QT) ? 2 : FCA_CPL2_WAVES) : ((QT) && !(K8) ? 3 : 4))

@@ -890,10 +890,11 @@ template <class R, int K8, int QT, class LP>
 FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int c_first)
 {
     typedef WalkLane<R> L;
-    /* row accesses with a scalar base for the float K = 8 walk of one
-     * column per work-item: the vector registers the plain form spends on
-     * addresses are what keeps it from four wavefronts per SIMD */
-    constexpr bool RF = K8 && sizeof(R) == 4 && LP::WIDTH == 64;
+    /* row accesses with a scalar base for the K = 8 walk of one column per
+     * work-item: the vector registers the plain form spends on addresses are
+     * what keeps it from one more wavefront per SIMD (four in float, three in
+     * double) */
+    constexpr bool RF = K8 && LP::WIDTH == 64;
     const int nr = a.nr, nc = a.nc;
     const int rb = (r_out0 - 3 > 0) ? r_out0 - 3 : 0;
     const int re = (r_out1 + 2 < nr + 2) ? r_out1 + 2 : nr + 2; /* inclusive */
