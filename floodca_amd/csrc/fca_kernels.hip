@@ -486,8 +486,8 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
  * fourth wavefront hides row-load latency, and so do both K=8 walks with
  * their row accesses on a scalar base (121-126 VGPRs; 136-140 with plain
  * global accesses, which ran three); the K=4 walk with the bookkeeping runs
- * three (134 VGPRs; on a scalar base it fits four, but ran up to 13% slower
- * on wet grids). Float, two columns per work-item: about 170 VGPRs for K=4
+ * three (134 VGPRs; at four, on a scalar base or with 8 VGPRs spilled, it
+ * ran 13% slower on a grid 20% wet). Float, two columns per work-item: about 170 VGPRs for K=4
  * (three wavefronts, FCA_CPL2_WAVES) and more with K=8 or the bookkeeping
  * (two). Double, one column per work-item: three for K=4 (163-165 VGPRs) and
  * for K=8 with its row accesses on a scalar base (164-168; 179-184 with
