@@ -147,9 +147,13 @@ struct WalkLane {
  * vector registers and adds the row's offset with one vector add per access.
  * With SB true, the row's address is pinned to two scalar registers and boff
  * passed as the access's 32-bit vector offset: no vector register holds an
- * address, at the cost of scalar work per row. The plain form is the faster
- * wherever the walk fits its occupancy anyway; the scalar base frees the
- * registers the float K = 8 walk needs to run four wavefronts per SIMD. */
+ * address, at the cost of scalar work per row. (The empty asm statements pin
+ * the row to scalar registers and hide where boff comes from, so that the
+ * compiler cannot hoist a 64-bit lane address out of the row loop; the cast
+ * to the global address space restores what the asm hides, without which the
+ * access would be a flat one.) The plain form is the faster wherever the walk
+ * fits its occupancy anyway; the scalar base frees the registers the K = 8
+ * walks need to run one more wavefront per SIMD. */
 template <bool SB, class T>
 FCA_HDI T ld_row(const T *row, uint32_t boff)
 {
@@ -894,7 +898,7 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
      * work-item: the vector registers the plain form spends on addresses are
      * what keeps it from one more wavefront per SIMD (four in float, three in
      * double) */
-    constexpr bool RF = K8 && LP::WIDTH == 64;
+    constexpr bool SB = K8 && LP::WIDTH == 64;
     const int nr = a.nr, nc = a.nc;
     const int rb = (r_out0 - 3 > 0) ? r_out0 - 3 : 0;
     const int re = (r_out1 + 2 < nr + 2) ? r_out1 + 2 : nr + 2; /* inclusive */
@@ -931,9 +935,9 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             const size_t ro = (size_t)rb * (size_t)nc;
             s.pm1 = a.mask[ro + (uint32_t)s.c];
             if (s.pm1 == FCA_LAND) {
-                s.pz = ld_row<RF>(a.z + ro, s.boff);
-                s.ph = ld_row<RF>(a.h_in + ro, s.boff);
-                s.pn = ld_row<RF>(a.n + ro, s.boff);
+                s.pz = ld_row<SB>(a.z + ro, s.boff);
+                s.ph = ld_row<SB>(a.h_in + ro, s.boff);
+                s.pn = ld_row<SB>(a.n + ro, s.boff);
             }
         }
         if (s.colok && rb + 1 < nr) {
@@ -978,25 +982,25 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
             lp.loads_settled();
             lp.each([&](L &s, int) {
                 if (s.stq) {
-                    st_row<RF>(hos, s.boff, s.hn3);
+                    st_row<SB>(hos, s.boff, s.hn3);
                     if (s.stq & 2) {
-                        st_row<RF>(hms, s.boff, s.hst);
-                        st_row<RF>(tms, s.boff, tval);
+                        st_row<SB>(hms, s.boff, s.hst);
+                        st_row<SB>(tms, s.boff, tval);
                     }
                     s.stq = 0;
                 }
                 s.pm1 = s.pm2;
                 if (s.colok && ok1 && s.pm1 == FCA_LAND) {
-                    s.pz = ld_row<RF>(zr, s.boff), s.ph = ld_row<RF>(hr, s.boff);
-                    s.pn = ld_row<RF>(nrw, s.boff);
+                    s.pz = ld_row<SB>(zr, s.boff), s.ph = ld_row<SB>(hr, s.boff);
+                    s.pn = ld_row<SB>(nrw, s.boff);
                 }
-                s.pm2 = (s.colok && ok2) ? (int)ld_row<RF>(mr, (uint32_t)s.c) : FCA_NONE;
+                s.pm2 = (s.colok && ok2) ? (int)ld_row<SB>(mr, (uint32_t)s.c) : FCA_NONE;
                 /* cr and hmax of row r-1 (m1), updated one row from now;
                  * hmax is read before that update writes it */
                 if (s.colok && okq && s.m1 == FCA_LAND) {
-                    s.pcr1 = ld_row<RF>(crr, s.boff);
+                    s.pcr1 = ld_row<SB>(crr, s.boff);
                     if (s.outcol && outq) {
-                        s.phm1 = ld_row<RF>(hmr, s.boff);
+                        s.phm1 = ld_row<SB>(hmr, s.boff);
                     }
                 }
             });
@@ -1092,10 +1096,10 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
         R *hos = a.h_out + ros, *hms = a.hmax + ros, *tms = a.tmax + ros;
         lp.each([&](L &s, int) {
             if (s.stq) {
-                st_row<RF>(hos, s.boff, s.hn3);
+                st_row<SB>(hos, s.boff, s.hn3);
                 if (s.stq & 2) {
-                    st_row<RF>(hms, s.boff, s.hst);
-                    st_row<RF>(tms, s.boff, a.sv.tmax_prev);
+                    st_row<SB>(hms, s.boff, s.hst);
+                    st_row<SB>(tms, s.boff, a.sv.tmax_prev);
                 }
                 s.stq = 0;
             }
