@@ -483,23 +483,25 @@ __device__ __forceinline__ void fused_body(const FusedArgs &a, int wg, int nwg)
 /* Wavefronts per SIMD the walk is compiled for, each the most that compiles
  * without spilling to scratch. Float, one column per work-item: the K=4 walk
  * without the quiet-tile bookkeeping fits four (at most 128 VGPRs), whose
- * fourth wavefront hides row-load latency, and so do both K=8 walks with
- * their row accesses on a scalar base (121-126 VGPRs; 136-140 with plain
- * global accesses, which ran three); the K=4 walk with the bookkeeping runs
- * three (134 VGPRs; at four, on a scalar base or with 8 VGPRs spilled, it
- * ran 13% slower on a grid 20% wet). Float, two columns per work-item: about 170 VGPRs for K=4
+ * fourth wavefront hides row-load latency, and so does the K=8 walk with
+ * its row accesses on a scalar base (121 VGPRs; 136 with plain global
+ * accesses, which ran three). The walks with the bookkeeping run three (K=4
+ * 134 VGPRs, K=8 140): at four (a scalar base, or for K=4 also 8 VGPRs
+ * spilled) they ran 13% (K=4) and 19% (K=8) slower on a grid 20% wet. Float, two columns per work-item: about 170 VGPRs for K=4
  * (three wavefronts, FCA_CPL2_WAVES) and more with K=8 or the bookkeeping
  * (two). Double, one column per work-item: three for K=4 (163-165 VGPRs) and
- * for K=8 with its row accesses on a scalar base (164-168; 179-184 with
- * plain global accesses, which ran two); its two-column form is not built. */
+ * for K=8 without the bookkeeping, its row accesses on a scalar base (164;
+ * 179 with plain global accesses, which ran two); two for K=8 with the
+ * bookkeeping (184 VGPRs, plain accesses as in float). Its two-column form is
+ * not built. */
 #ifndef FCA_CPL2_WAVES
 #define FCA_CPL2_WAVES 3
 #endif
 #ifdef FCA_DOUBLE
-#define FCA_FUSED_WAVES(K8, QT, CPL) 3
+#define FCA_FUSED_WAVES(K8, QT, CPL) ((K8) && (QT) ? 2 : 3)
 #else
 #define FCA_FUSED_WAVES(K8, QT, CPL) \
-    ((CPL) == 2 ? ((K8) || (QT) ? 2 : FCA_CPL2_WAVES) : ((QT) && !(K8) ? 3 : 4))
+    ((CPL) == 2 ? ((K8) || (QT) ? 2 : FCA_CPL2_WAVES) : ((QT) ? 3 : 4))
 #endif
 #define FCA_FUSED_OCCUPANCY(K8, QT, CPL) \
     __attribute__((amdgpu_waves_per_eu(FCA_FUSED_WAVES(K8, QT, CPL))))

@@ -895,10 +895,12 @@ FCA_HD void walk_strip(LP &lp, const WalkArgs<R> &a, int r_out0, int r_out1, int
 {
     typedef WalkLane<R> L;
     /* row accesses with a scalar base for the K = 8 walk of one column per
-     * work-item: the vector registers the plain form spends on addresses are
-     * what keeps it from one more wavefront per SIMD (four in float, three in
-     * double) */
-    constexpr bool SB = K8 && LP::WIDTH == 64;
+     * work-item without the quiet-tile bookkeeping: the vector registers the
+     * plain form spends on addresses are what keeps it from one more
+     * wavefront per SIMD (four in float, three in double). With the
+     * bookkeeping the extra wavefront loses on wet grids, as for K = 4, so
+     * that walk keeps the plain form. */
+    constexpr bool SB = K8 && !QT && LP::WIDTH == 64;
     const int nr = a.nr, nc = a.nc;
     const int rb = (r_out0 - 3 > 0) ? r_out0 - 3 : 0;
     const int re = (r_out1 + 2 < nr + 2) ? r_out1 + 2 : nr + 2; /* inclusive */
